@@ -1,0 +1,1171 @@
+// HIP kernels for the Fourier-domain projection gridder, written for gfx950
+// (CDNA4, wave64).
+//
+// Numerics contract.  This file is compiled with -ffp-contract=off: every
+// FP32 expression that feeds a discrete decision (rotated coordinates, roundf
+// of them, kernel-table indices, the mask and slab tests) is evaluated with
+// separately rounded operations in the order the reference kernels write
+// them, so rounding decisions match the CPU oracle bit for bit.  Fused
+// multiply-adds appear only where written explicitly (fmaf), and only in
+// accumulations.
+//
+// Layout: column-major, x fastest.  vol[x + y*V + z*V*V];
+// CAS image a: img[a*M*M + i + j*M]; a direction is 9 floats [u; v; n].
+#include "mgg_internal.h"
+
+#include <hip/hip_fp16.h>
+
+namespace mgg {
+namespace {
+
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u3 __attribute__((ext_vector_type(3)));
+
+constexpr int kThreads = 256;
+
+// ---------------------------------------------------------------------------
+// Kernel-table index (gpuForwardProject.cu:63-67, gpuBackProject.cu:79-82):
+// clamp to [-convW, convW], scale, round half away from zero, clamp to table.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int kb_index(float r, float fW, float kerScale, float kerCenter,
+                                        int K)
+{
+    r = fminf(fmaxf(r, -fW), fW);
+    float s = r * kerScale;
+    s = s + kerCenter;
+    int idx = (int)roundf(s);
+    idx = min(max(idx, 0), K - 1);
+    return idx;
+}
+
+// XCD-aware work mapping: hardware hands consecutive workgroup ids to the 8
+// XCDs round-robin; remapping gives each XCD a contiguous run of work items so
+// neighbouring tiles of one direction share that XCD's L2.
+__device__ __forceinline__ unsigned xcd_work(unsigned perXcd)
+{
+    unsigned b = blockIdx.x;
+    return (b & 7u) * perXcd + (b >> 3);
+}
+
+// ---------------------------------------------------------------------------
+// Volume loads for the three storage types.  row5 fetches the 5 contiguous
+// x-taps of one (y, z) row with unaligned vector loads.
+// ---------------------------------------------------------------------------
+template <int DT> struct VolLoad;
+
+template <> struct VolLoad<DT_F32> {
+    typedef float T;
+    static constexpr int kRowSlack = 0;  // elements read past the 5th tap
+    static __device__ __forceinline__ float one(const float *p) { return *p; }
+    static __device__ __forceinline__ void row5(const float *p, float (&r)[5])
+    {
+        f4 a;
+        __builtin_memcpy(&a, p, 16);
+        r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+        r[4] = p[4];
+    }
+};
+
+template <> struct VolLoad<DT_BF16> {
+    typedef uint16_t T;
+    static constexpr int kRowSlack = 1;
+    static __device__ __forceinline__ float one(const uint16_t *p)
+    {
+        return __uint_as_float((uint32_t)(*p) << 16);
+    }
+    static __device__ __forceinline__ void row5(const uint16_t *p, float (&r)[5])
+    {
+        u3 a;
+        __builtin_memcpy(&a, p, 12);
+        r[0] = __uint_as_float(a.x << 16); r[1] = __uint_as_float(a.x & 0xffff0000u);
+        r[2] = __uint_as_float(a.y << 16); r[3] = __uint_as_float(a.y & 0xffff0000u);
+        r[4] = __uint_as_float(a.z << 16);
+    }
+};
+
+template <> struct VolLoad<DT_F16> {
+    typedef uint16_t T;
+    static constexpr int kRowSlack = 1;
+    static __device__ __forceinline__ float h2f(uint32_t bits)
+    {
+        __half_raw raw;
+        raw.x = (unsigned short)(bits & 0xffffu);
+        return __half2float(__half(raw));
+    }
+    static __device__ __forceinline__ float one(const uint16_t *p) { return h2f(*p); }
+    static __device__ __forceinline__ void row5(const uint16_t *p, float (&r)[5])
+    {
+        u3 a;
+        __builtin_memcpy(&a, p, 12);
+        r[0] = h2f(a.x); r[1] = h2f(a.x >> 16);
+        r[2] = h2f(a.y); r[3] = h2f(a.y >> 16);
+        r[4] = h2f(a.z);
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Forward projection (gpuForwardProject.cu:4-101).
+// Work item = (direction, 16x16 pixel tile); one pixel per thread; wave = 16x4
+// pixels.  The KB table sits in LDS.  For kerHWidth 2 the 5x5x5 gather is
+// unrolled with one unaligned vector load per x-row; other widths use the
+// generic loop.
+// ---------------------------------------------------------------------------
+struct FPArgs {
+    const void *vol;
+    const float *axes;
+    float *out;
+    const float *ker;
+    int V, M, K, nAxes;
+    float maskR, kerScale, kerCenter, fW;
+    int convW, tilesPerRow, tilesPerImg;
+    unsigned perXcd, totalWork;
+};
+
+template <int DT, bool W2>
+__global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
+{
+    using L = VolLoad<DT>;
+    using T = typename L::T;
+    __shared__ float sker[kMaxKerSize];
+
+    const unsigned work = xcd_work(p.perXcd);
+    if (work >= p.totalWork) return;  // uniform per block
+    for (int t = threadIdx.x; t < p.K; t += kThreads) sker[t] = p.ker[t];
+    __syncthreads();
+
+    const int a = (int)(work / (unsigned)p.tilesPerImg);
+    const int tile = (int)(work - (unsigned)a * (unsigned)p.tilesPerImg);
+    const int ty = tile / p.tilesPerRow, tx = tile - ty * p.tilesPerRow;
+    const int i = tx * 16 + (threadIdx.x & 15), j = ty * 16 + (threadIdx.x >> 4);
+    const int M = p.M, V = p.V;
+    if (i >= M || j >= M) return;
+    const int ci = M / 2, cv = V / 2;
+
+    float r = sqrtf((float)(i - ci) * (i - ci) + (j - ci) * (j - ci));
+    float acc = 0.f;
+    if (r <= p.maskR) {
+        const float *ax = p.axes + (size_t)9 * a;
+        const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
+        const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
+        const float fi = (float)(i - ci), fj = (float)(j - ci);
+        const float x = u0 * fi + v0 * fj + (float)cv;
+        const float y = u1 * fi + v1 * fj + (float)cv;
+        const float z = u2 * fi + v2 * fj + (float)cv;
+        const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
+        const T *vol = (const T *)p.vol;
+        const long long V3 = (long long)V * V * V;
+
+        if (W2) {
+            float wi[5], wj[5], wk[5];
+#pragma unroll
+            for (int t = 0; t < 5; t++) {
+                wi[t] = sker[kb_index((float)(X - 2 + t) - x, p.fW, p.kerScale, p.kerCenter, p.K)];
+                wj[t] = sker[kb_index((float)(Y - 2 + t) - y, p.fW, p.kerScale, p.kerCenter, p.K)];
+                wk[t] = sker[kb_index((float)(Z - 2 + t) - z, p.fW, p.kerScale, p.kerCenter, p.K)];
+            }
+            float wij[5][5];
+#pragma unroll
+            for (int aa = 0; aa < 5; aa++)
+#pragma unroll
+                for (int bb = 0; bb < 5; bb++) wij[aa][bb] = wi[aa] * wj[bb];
+
+            const bool fast = X >= 2 && X + 2 + L::kRowSlack < V && Y >= 2 && Y + 2 < V &&
+                              Z >= 2 && Z + 2 < V;
+            if (fast) {
+                const T *base = vol + (((size_t)(Z - 2) * V + (size_t)(Y - 2)) * V + (X - 2));
+#pragma unroll
+                for (int cc = 0; cc < 5; cc++) {
+#pragma unroll
+                    for (int bb = 0; bb < 5; bb++) {
+                        float rv[5];
+                        L::row5(base + ((size_t)cc * V + bb) * V, rv);
+#pragma unroll
+                        for (int aa = 0; aa < 5; aa++) acc = fmaf(wij[aa][bb] * wk[cc], rv[aa], acc);
+                    }
+                }
+            } else {
+                // Neighbourhood leaves the volume (only with a user mask radius
+                // beyond the default or non-unit axes): the reference's
+                // linear-index test (gpuForwardProject.cu:90), plus a lower
+                // bound so nothing is read before the volume.
+#pragma unroll
+                for (int aa = 0; aa < 5; aa++)
+#pragma unroll
+                    for (int bb = 0; bb < 5; bb++)
+#pragma unroll
+                        for (int cc = 0; cc < 5; cc++) {
+                            long long lin = (long long)(Z - 2 + cc) * V * V +
+                                            (long long)(Y - 2 + bb) * V + (X - 2 + aa);
+                            if (lin < V3 && lin >= 0)
+                                acc = fmaf(wij[aa][bb] * wk[cc], L::one(vol + lin), acc);
+                        }
+            }
+        } else {
+            const int W = p.convW;
+            for (int i1 = X - W; i1 <= X + W; i1++) {
+                float wi = sker[kb_index((float)i1 - x, p.fW, p.kerScale, p.kerCenter, p.K)];
+                for (int j1 = Y - W; j1 <= Y + W; j1++) {
+                    float wj = sker[kb_index((float)j1 - y, p.fW, p.kerScale, p.kerCenter, p.K)];
+                    for (int k1 = Z - W; k1 <= Z + W; k1++) {
+                        float wk = sker[kb_index((float)k1 - z, p.fW, p.kerScale, p.kerCenter, p.K)];
+                        float w = wi * wj * wk;
+                        long long lin = (long long)k1 * V * V + (long long)j1 * V + i1;
+                        if (lin < V3 && lin >= 0) acc = fmaf(w, L::one(vol + lin), acc);
+                    }
+                }
+            }
+        }
+    }
+    p.out[(size_t)a * M * M + (size_t)j * M + i] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// Exact transpose of the forward gather: each in-mask pixel scatters its
+// value through the same (voxel, weight) pairs with FP32 atomics.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kThreads) void adj_kernel(FPArgs p, const float *__restrict__ img,
+                                                       float *__restrict__ volAcc)
+{
+    __shared__ float sker[kMaxKerSize];
+    const unsigned work = xcd_work(p.perXcd);
+    if (work >= p.totalWork) return;
+    for (int t = threadIdx.x; t < p.K; t += kThreads) sker[t] = p.ker[t];
+    __syncthreads();
+
+    const int a = (int)(work / (unsigned)p.tilesPerImg);
+    const int tile = (int)(work - (unsigned)a * (unsigned)p.tilesPerImg);
+    const int ty = tile / p.tilesPerRow, tx = tile - ty * p.tilesPerRow;
+    const int i = tx * 16 + (threadIdx.x & 15), j = ty * 16 + (threadIdx.x >> 4);
+    const int M = p.M, V = p.V;
+    if (i >= M || j >= M) return;
+    const int ci = M / 2, cv = V / 2;
+    float r = sqrtf((float)(i - ci) * (i - ci) + (j - ci) * (j - ci));
+    if (!(r <= p.maskR)) return;
+    const float val = img[(size_t)a * M * M + (size_t)j * M + i];
+    if (val == 0.f) return;
+    const float *ax = p.axes + (size_t)9 * a;
+    const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
+    const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
+    const float fi = (float)(i - ci), fj = (float)(j - ci);
+    const float x = u0 * fi + v0 * fj + (float)cv;
+    const float y = u1 * fi + v1 * fj + (float)cv;
+    const float z = u2 * fi + v2 * fj + (float)cv;
+    const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
+    const long long V3 = (long long)V * V * V;
+    const int W = p.convW;
+    for (int i1 = X - W; i1 <= X + W; i1++) {
+        float wi = sker[kb_index((float)i1 - x, p.fW, p.kerScale, p.kerCenter, p.K)];
+        for (int j1 = Y - W; j1 <= Y + W; j1++) {
+            float wj = sker[kb_index((float)j1 - y, p.fW, p.kerScale, p.kerCenter, p.K)];
+            for (int k1 = Z - W; k1 <= Z + W; k1++) {
+                float wk = sker[kb_index((float)k1 - z, p.fW, p.kerScale, p.kerCenter, p.K)];
+                float w = wi * wj * wk;
+                long long lin = (long long)k1 * V * V + (long long)j1 * V + i1;
+                if (lin < V3 && lin >= 0 && w != 0.f) unsafeAtomicAdd(volAcc + lin, w * val);
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Back projection, reference semantics (gpuBackProject.cu:4-113), as an
+// owner-computes gather: a workgroup owns an 8x8x4 voxel brick, culls the
+// directions whose slab |n.(v-c)| <= kerHWidth cannot touch the brick, keeps
+// the survivors in LDS in direction order, and each thread accumulates its own
+// voxel.  No atomics: launches on one volume must be stream-ordered.
+// ---------------------------------------------------------------------------
+struct BPArgs {
+    float *vol;
+    float *dens;
+    const float *axes;
+    const float *img;
+    const float *ker;
+    int V, M, K, nAxes;
+    float maskR, kerScale, kerCenter, fW, hw;
+    int convW, bricksX, bricksXY;
+    unsigned perXcd, totalWork;
+};
+
+constexpr int BPX = 8, BPY = 8, BPZ = 4;
+
+template <bool DENS>
+__global__ __launch_bounds__(kThreads) void bp_kernel(BPArgs p)
+{
+    __shared__ float sker[kMaxKerSize];
+    __shared__ float sax[kThreads * 9];
+    __shared__ int sidx[kThreads];
+    __shared__ int swave[kThreads / 64];
+
+    const unsigned work = xcd_work(p.perXcd);
+    if (work >= p.totalWork) return;
+    const int bz = (int)(work / (unsigned)p.bricksXY);
+    const int rem = (int)(work - (unsigned)bz * (unsigned)p.bricksXY);
+    const int by = rem / p.bricksX, bx = rem - by * p.bricksX;
+    const int ox = bx * BPX, oy = by * BPY, oz = bz * BPZ;
+    const int V = p.V, M = p.M, cv = V / 2, ci = M / 2;
+
+    // Whole brick outside the mask sphere: nothing to do (uniform exit).
+    {
+        float dx = fmaxf(fmaxf((float)(ox - cv), (float)(cv - (ox + BPX - 1))), 0.f);
+        float dy = fmaxf(fmaxf((float)(oy - cv), (float)(cv - (oy + BPY - 1))), 0.f);
+        float dz = fmaxf(fmaxf((float)(oz - cv), (float)(cv - (oz + BPZ - 1))), 0.f);
+        if (dx * dx + dy * dy + dz * dz > (p.maskR + 1.f) * (p.maskR + 1.f)) return;
+    }
+    const int t = threadIdx.x;
+    for (int k = t; k < p.K; k += kThreads) sker[k] = p.ker[k];
+
+    const int vi = ox + (t & 7), vj = oy + ((t >> 3) & 7), vk = oz + (t >> 6);
+    const bool inVol = vi < V && vj < V && vk < V;
+    const float rr = sqrtf((float)(vi - cv) * (vi - cv) + (vj - cv) * (vj - cv) + (vk - cv) * (vk - cv));
+    const bool active = inVol && (rr <= p.maskR);
+    const float dvi = (float)vi - cv, dvj = (float)vj - cv, dvk = (float)vk - cv;
+    const float fvi = (float)vi, fvj = (float)vj, fvk = (float)vk, cvf = (float)cv;
+    const float bcx = (float)ox + 0.5f * (BPX - 1) - (float)cv;
+    const float bcy = (float)oy + 0.5f * (BPY - 1) - (float)cv;
+    const float bcz = (float)oz + 0.5f * (BPZ - 1) - (float)cv;
+    const int M2 = M * M;
+
+    double accAll = 0.0, densAll = 0.0;
+    const int wv = t >> 6, ln = t & 63;
+
+    for (int base = 0; base < p.nAxes; base += kThreads) {
+        const int n = min(kThreads, p.nAxes - base);
+        __syncthreads();  // previous list consumed (and table loaded)
+        bool pass = false;
+        float a9[9];
+        if (t < n) {
+            const float *ax = p.axes + (size_t)9 * (base + t);
+#pragma unroll
+            for (int k = 0; k < 9; k++) a9[k] = ax[k];
+            const float d = bcx * a9[6] + bcy * a9[7] + bcz * a9[8];
+            const float sup = 0.5f * (BPX - 1) * fabsf(a9[6]) + 0.5f * (BPY - 1) * fabsf(a9[7]) +
+                              0.5f * (BPZ - 1) * fabsf(a9[8]);
+            pass = fabsf(d) <= p.hw + sup + 1e-3f;
+        }
+        const unsigned long long bal = __ballot(pass);
+        if (ln == 0) swave[wv] = __popcll(bal);
+        __syncthreads();
+        int off = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; w++) {
+            off += (w < wv) ? swave[w] : 0;
+            total += swave[w];
+        }
+        if (pass) {
+            const int pos = off + __popcll(bal & ((1ull << ln) - 1ull));
+            sidx[pos] = base + t;
+#pragma unroll
+            for (int k = 0; k < 9; k++) sax[pos * 9 + k] = a9[k];
+        }
+        __syncthreads();
+
+        for (int q = 0; q < total; q++) {
+            const float *ax = sax + q * 9;
+            const float n0 = ax[6], n1 = ax[7], n2 = ax[8];
+            const float fk = dvi * n0 + dvj * n1 + dvk * n2;
+            if (!(active && fabsf(fk) <= p.hw)) continue;
+            const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
+            const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
+            const float fi = dvi * u0 + dvj * u1 + dvk * u2 + (float)ci;
+            const float fj = dvi * v0 + dvj * v1 + dvk * v2 + (float)ci;
+            const int I = (int)roundf(fi), J = (int)roundf(fj);
+            const float *im = p.img + (size_t)sidx[q] * M2;
+            float cum = 0.f, dsum = 0.f;
+            if (p.convW == 2) {
+                float Ax[5], Ay[5], Az[5], Bx[5], By[5], Bz[5];
+#pragma unroll
+                for (int s = 0; s < 5; s++) {
+                    const float di = (float)(I - 2 + s - ci), dj = (float)(J - 2 + s - ci);
+                    Ax[s] = di * u0; Ay[s] = di * u1; Az[s] = di * u2;
+                    Bx[s] = dj * v0; By[s] = dj * v1; Bz[s] = dj * v2;
+                }
+#pragma unroll
+                for (int jj = 0; jj < 5; jj++) {
+#pragma unroll
+                    for (int ii = 0; ii < 5; ii++) {
+                        const float px = Ax[ii] + Bx[jj] + cvf;
+                        float w = sker[kb_index(px - fvi, p.fW, p.kerScale, p.kerCenter, p.K)];
+                        const float py = Ay[ii] + By[jj] + cvf;
+                        w = w * sker[kb_index(py - fvj, p.fW, p.kerScale, p.kerCenter, p.K)];
+                        const float pz = Az[ii] + Bz[jj] + cvf;
+                        w = w * sker[kb_index(pz - fvk, p.fW, p.kerScale, p.kerCenter, p.K)];
+                        const int lin = (J - 2 + jj) * M + (I - 2 + ii);
+                        if (lin >= 0 && lin < M2) {
+                            cum = cum + im[lin] * w;
+                            if (DENS) dsum = dsum + w;
+                        }
+                    }
+                }
+            } else {
+                const int W = p.convW;
+                for (int j1 = J - W; j1 <= J + W; j1++) {
+                    for (int i1 = I - W; i1 <= I + W; i1++) {
+                        const float px = (float)(i1 - ci) * u0 + (float)(j1 - ci) * v0 + cvf;
+                        float w = sker[kb_index(px - fvi, p.fW, p.kerScale, p.kerCenter, p.K)];
+                        const float py = (float)(i1 - ci) * u1 + (float)(j1 - ci) * v1 + cvf;
+                        w = w * sker[kb_index(py - fvj, p.fW, p.kerScale, p.kerCenter, p.K)];
+                        const float pz = (float)(i1 - ci) * u2 + (float)(j1 - ci) * v2 + cvf;
+                        w = w * sker[kb_index(pz - fvk, p.fW, p.kerScale, p.kerCenter, p.K)];
+                        const int lin = j1 * M + i1;
+                        if (lin >= 0 && lin < M2) {
+                            cum = cum + im[lin] * w;
+                            if (DENS) dsum = dsum + w;
+                        }
+                    }
+                }
+            }
+            accAll += (double)cum;
+            if (DENS) densAll += (double)dsum;
+        }
+    }
+    if (active) {
+        const size_t vidx = ((size_t)vk * V + vj) * V + vi;
+        p.vol[vidx] = (float)((double)p.vol[vidx] + accAll);
+        if (DENS) p.dens[vidx] = (float)((double)p.dens[vidx] + densAll);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Image chains.  With even M, fftshift o DFT o fftshift equals the plain DFT
+// with a (-1)^(x+y) checkerboard on input and output (the 2-D global sign
+// (-1)^M is +1), so the shifts of gpuGridder.cpp:463-488 and :562-587 fold
+// into these loads and stores.  The CAS-derived spectrum is exactly Hermitian,
+// so the inverse uses C2R and the forward R2C (half the FFT work).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ float chk(int s) { return (s & 1) ? -1.f : 1.f; }
+
+// CASToComplexFilter.cu:53-75 (2-D stack mode) + fftshift, half spectrum.
+__global__ void k_cas_to_herm2d(const float *__restrict__ cas, float2 *__restrict__ herm, int M,
+                                size_t total)
+{
+    const int H = M / 2 + 1;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int cx = (int)(e % H);
+        const size_t rest = e / H;
+        const int cy = (int)(rest % M);
+        const size_t a = rest / M;
+        float2 z = make_float2(0.f, 0.f);
+        if (cx != 0 && cy != 0) {
+            const float *c = cas + a * M * M;
+            const float c1 = c[(size_t)cy * M + cx];
+            const float c2 = c[(size_t)(M - cy) * M + (M - cx)];
+            const float s = chk(cx + cy);
+            z.x = s * (0.5f * (c1 + c2));
+            z.y = s * (0.5f * (c1 - c2));
+        }
+        herm[e] = z;
+    }
+}
+
+// fftshift + ComplexToReal + CropVolume + DivideScalar (gpuGridder.cpp:485-517).
+__global__ void k_real2d_to_imgs(const float *__restrict__ real, float *__restrict__ imgs, int M,
+                                 int N, float scale, size_t total)
+{
+    const int off = (M - N) / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int px = (int)(e % N);
+        const size_t rest = e / N;
+        const int py = (int)(rest % N);
+        const size_t a = rest / N;
+        const int mx = px + off, my = py + off;
+        imgs[e] = chk(mx + my) * real[a * M * M + (size_t)my * M + mx] * scale;
+    }
+}
+
+// PadVolume + RealToComplex + fftshift (gpuGridder.cpp:543-566).
+__global__ void k_imgs_to_real2d(const float *__restrict__ imgs, float *__restrict__ real, int M,
+                                 int N, size_t total)
+{
+    const int off = (M - N) / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int mx = (int)(e % M);
+        const size_t rest = e / M;
+        const int my = (int)(rest % M);
+        const size_t a = rest / M;
+        const int px = mx - off, py = my - off;
+        float v = 0.f;
+        if (px >= 0 && px < N && py >= 0 && py < N)
+            v = chk(mx + my) * imgs[a * N * N + (size_t)py * N + px];
+        real[e] = v;
+    }
+}
+
+// fftshift + ComplexToCAS (gpuGridder.cpp:584-595) from the R2C half spectrum.
+__global__ void k_herm2d_to_cas(const float2 *__restrict__ herm, float *__restrict__ cas, int M,
+                                size_t total)
+{
+    const int H = M / 2 + 1;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int cx = (int)(e % M);
+        const size_t rest = e / M;
+        const int cy = (int)(rest % M);
+        const size_t a = rest / M;
+        const float2 *h = herm + a * (size_t)M * H;
+        float re, im;
+        if (cx <= M / 2) {
+            const float2 f = h[(size_t)cy * H + cx];
+            re = f.x; im = f.y;
+        } else {
+            const float2 f = h[(size_t)((M - cy) % M) * H + (M - cx)];
+            re = f.x; im = -f.y;
+        }
+        const float s = chk(cx + cy);
+        cas[e] = s * re + s * im;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Volume chains (utils/CASFromVol_Gridder.m:34-60, utils/volFromCAS_Gridder.m:
+// 37-56).  In 3-D the shift identity carries a global sign (-1)^(3M/2).
+// ---------------------------------------------------------------------------
+__global__ void k_vol_to_real3d(const float *__restrict__ vol, int N, float *__restrict__ real,
+                                int M, const float *__restrict__ pc, size_t total)
+{
+    const int off = (M - N) / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int mx = (int)(e % M);
+        const size_t rest = e / M;
+        const int my = (int)(rest % M);
+        const int mz = (int)(rest / M);
+        const int qx = mx - off, qy = my - off, qz = mz - off;
+        float v = 0.f;
+        if (qx >= 0 && qx < N && qy >= 0 && qy < N && qz >= 0 && qz < N) {
+            const double w = (double)pc[mx] * (double)pc[my] * (double)pc[mz];
+            v = chk(mx + my + mz) *
+                (float)(w * (double)vol[((size_t)qz * N + qy) * N + qx]);
+        }
+        real[e] = v;
+    }
+}
+
+__global__ void k_herm3d_to_casvol(const float2 *__restrict__ herm, int M,
+                                   float *__restrict__ casVol, int V, int pad, float gsign,
+                                   size_t total)
+{
+    const int H = M / 2 + 1;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int vx = (int)(e % V);
+        const size_t rest = e / V;
+        const int vy = (int)(rest % V);
+        const int vz = (int)(rest / V);
+        const int cx = vx - pad, cy = vy - pad, cz = vz - pad;
+        float out = 0.f;
+        if (cx >= 0 && cx < M && cy >= 0 && cy < M && cz >= 0 && cz < M) {
+            float re, im;
+            if (cx <= M / 2) {
+                const float2 f = herm[((size_t)cz * M + cy) * H + cx];
+                re = f.x; im = f.y;
+            } else {
+                const float2 f = herm[((size_t)((M - cz) % M) * M + (M - cy) % M) * H + (M - cx)];
+                re = f.x; im = -f.y;
+            }
+            const float s = gsign * chk(cx + cy + cz);
+            out = s * re + s * im;
+        }
+        casVol[e] = out;
+    }
+}
+
+__global__ void k_casvol_to_herm3d(const float *__restrict__ casVol, int V, int pad,
+                                   float2 *__restrict__ herm, int M, int zeroNyq, size_t total)
+{
+    const int H = M / 2 + 1;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int cx = (int)(e % H);
+        const size_t rest = e / H;
+        const int cy = (int)(rest % M);
+        const int cz = (int)(rest / M);
+        float2 z = make_float2(0.f, 0.f);
+        if (!(zeroNyq && (cx == 0 || cy == 0 || cz == 0))) {
+            const float c1 = casVol[((size_t)(cz + pad) * V + (cy + pad)) * V + (cx + pad)];
+            const int mx = (M - cx) % M, my = (M - cy) % M, mz = (M - cz) % M;
+            const float c2 = casVol[((size_t)(mz + pad) * V + (my + pad)) * V + (mx + pad)];
+            const float s = chk(cx + cy + cz);
+            z.x = s * (0.5f * (c1 + c2));
+            z.y = s * (0.5f * (c1 - c2));
+        }
+        herm[e] = z;
+    }
+}
+
+__global__ void k_real3d_to_vol(const float *__restrict__ real, int M, float *__restrict__ vol,
+                                int N, const float *__restrict__ pc, double scale, size_t total)
+{
+    const int off = (M - N) / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int qx = (int)(e % N);
+        const size_t rest = e / N;
+        const int qy = (int)(rest % N);
+        const int qz = (int)(rest / N);
+        const int mx = qx + off, my = qy + off, mz = qz + off;
+        const double w = (double)pc[mx] * (double)pc[my] * (double)pc[mz] * scale;
+        vol[e] = chk(mx + my + mz) * (float)(w * (double)real[((size_t)mz * M + my) * M + mx]);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions, conversions, fills.
+// ---------------------------------------------------------------------------
+__global__ void k_reduce_partials(float *__restrict__ out, const float *__restrict__ parts,
+                                  int nParts, size_t n, size_t stride, int accumulate)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        double s = accumulate ? (double)out[e] : 0.0;
+        for (int q = 0; q < nParts; q++) s += (double)parts[(size_t)q * stride + e];
+        out[e] = (float)s;
+    }
+}
+
+__global__ void k_divide_density(float *__restrict__ out, const float *__restrict__ num,
+                                 const float *__restrict__ den, size_t n, double eps)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        out[e] = (float)((double)num[e] / ((double)den[e] + eps));
+}
+
+__global__ void k_to_bf16(uint16_t *__restrict__ dst, const float *__restrict__ src, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        uint32_t u = __float_as_uint(src[e]);
+        if ((u & 0x7f800000u) == 0x7f800000u) {  // inf / nan: truncate, keep nan quiet
+            dst[e] = (uint16_t)((u >> 16) | ((u & 0x007fffffu) ? 0x40u : 0u));
+        } else {
+            u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
+            dst[e] = (uint16_t)(u >> 16);
+        }
+    }
+}
+
+__global__ void k_to_f16(uint16_t *__restrict__ dst, const float *__restrict__ src, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        __half h = __float2half_rn(src[e]);
+        dst[e] = __half_raw(h).x;
+    }
+}
+
+__global__ void k_fill(float *__restrict__ p, float v, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        p[e] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Standalone filters (Filters/*.cu).  Same element semantics as the reference
+// kernels; grid-stride 1-D launches instead of one thread per (x, y) column.
+// ---------------------------------------------------------------------------
+__global__ void k_f_cas_to_complex(const float *__restrict__ in, float2 *__restrict__ out, int S,
+                                   int nSlices, int volumeMode, size_t total)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int i = (int)(e % S);
+        const size_t rest = e / S;
+        const int j = (int)(rest % S);
+        const int k = (int)(rest / S);
+        float2 z = make_float2(0.f, 0.f);
+        const bool skip = volumeMode ? (i == 0 || j == 0 || k == 0) : (i == 0 || j == 0);
+        if (!skip) {
+            const int k2 = volumeMode ? S - k : k;
+            const size_t n2 = (size_t)(S - i) + (size_t)(S - j) * S + (size_t)k2 * S * S;
+            z.x = 0.5f * (in[e] + in[n2]);
+            z.y = 0.5f * (in[e] - in[n2]);
+        }
+        out[e] = z;
+    }
+}
+
+__global__ void k_f_complex_to_cas(const float2 *__restrict__ in, float *__restrict__ out, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        out[e] = in[e].x + in[e].y;
+}
+
+__global__ void k_f_complex_to_real(const float2 *__restrict__ in, float *__restrict__ out, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        out[e] = in[e].x;
+}
+
+__global__ void k_f_real_to_complex(const float *__restrict__ in, float2 *__restrict__ out, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        out[e] = make_float2(in[e], 0.f);
+}
+
+// In-place quadrant swap per slice (even S): element (x, y) with y < S/2
+// swaps with ((x + S/2) mod S, y + S/2).
+template <typename T>
+__global__ void k_f_fftshift2d(T *__restrict__ d, int S, size_t total)
+{
+    const int h = S / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % S);
+        const size_t rest = e / S;
+        const int y = (int)(rest % h);
+        const size_t k = rest / h;
+        const size_t a = k * S * S + (size_t)y * S + x;
+        const size_t b = k * S * S + (size_t)(y + h) * S + ((x + h) % S);
+        T t = d[a];
+        d[a] = d[b];
+        d[b] = t;
+    }
+}
+
+// In-place octant swap (even S): element (x, y, z) with z < S/2 swaps with
+// ((x + S/2) mod S, (y + S/2) mod S, z + S/2).
+template <typename T>
+__global__ void k_f_fftshift3d(T *__restrict__ d, int S, size_t total)
+{
+    const int h = S / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % S);
+        const size_t rest = e / S;
+        const int y = (int)(rest % S);
+        const int z = (int)(rest / S);
+        const size_t a = ((size_t)z * S + y) * S + x;
+        const size_t b = ((size_t)(z + h) * S + (y + h) % S) * S + (x + h) % S;
+        T t = d[a];
+        d[a] = d[b];
+        d[b] = t;
+    }
+}
+
+__global__ void k_f_pad(const float *__restrict__ in, float *__restrict__ out, int inS, int outS,
+                        size_t total)
+{
+    const int off = (outS - inS) / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % outS);
+        const size_t rest = e / outS;
+        const int y = (int)(rest % outS);
+        const size_t k = rest / outS;
+        const int ix = x - off, iy = y - off;
+        float v = 0.f;
+        if (ix >= 0 && ix < inS && iy >= 0 && iy < inS) v = in[k * inS * inS + (size_t)iy * inS + ix];
+        out[e] = v;
+    }
+}
+
+template <typename T>
+__global__ void k_f_crop(const T *__restrict__ in, T *__restrict__ out, int inS, int outS,
+                         size_t total)
+{
+    const int off = (inS - outS) / 2;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const int x = (int)(e % outS);
+        const size_t rest = e / outS;
+        const int y = (int)(rest % outS);
+        const size_t k = rest / outS;
+        out[e] = in[k * inS * inS + (size_t)(y + off) * inS + (x + off)];
+    }
+}
+
+__global__ void k_f_divide_scalar(float *__restrict__ d, float s, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        d[e] = d[e] / s;
+}
+
+__global__ void k_f_divide_volume(float *__restrict__ a, const float *__restrict__ b, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        const float x = a[e], y = b[e];
+        a[e] = (x != 0.f && y != 0.f) ? x / y : 0.f;
+    }
+}
+
+__global__ void k_f_multiply_volume(float *__restrict__ a, const float *__restrict__ b, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        a[e] = a[e] * b[e];
+}
+
+__global__ void k_f_multiply_complex_volume(float2 *__restrict__ a, const float *__restrict__ b,
+                                            size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x) {
+        float2 z = a[e];
+        z.x = z.x * b[e];
+        z.y = z.y * b[e];
+        a[e] = z;
+    }
+}
+
+__global__ void k_f_add_volume(float *__restrict__ a, const float *__restrict__ b, size_t n)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        a[e] = a[e] + b[e];
+}
+
+// 1-D grid-stride launch geometry: enough blocks to fill 256 CUs many times
+// over, capped so huge arrays loop inside the kernel.
+inline unsigned grid_for(size_t n)
+{
+    size_t g = (n + kThreads - 1) / kThreads;
+    if (g > 65536) g = 65536;
+    if (g == 0) g = 1;
+    return (unsigned)g;
+}
+
+inline void proj_consts(const ProjGeom &g, float &kerScale, float &kerCenter, float &fW,
+                        int &convW)
+{
+    // gpuForwardProject.cu:20-23: kerCenter = (K-1)/2, kerScale = kerCenter/kerHWidth.
+    kerCenter = ((float)g.K - 1) / 2;
+    kerScale = kerCenter / g.kerHWidth;
+    convW = (int)roundf(g.kerHWidth);
+    fW = (float)convW;
+}
+
+}  // namespace
+
+// ===========================================================================
+// Launchers
+// ===========================================================================
+hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
+                                  const float *axes, int nAxes, float *casImgs, hipStream_t s)
+{
+    if (nAxes <= 0) return hipSuccess;
+    FPArgs a;
+    a.vol = vol; a.axes = axes; a.out = casImgs; a.ker = g.ker;
+    a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
+    a.maskR = g.maskRadius;
+    proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
+    a.tilesPerRow = (g.M + 15) / 16;
+    a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
+    const unsigned long long work = (unsigned long long)a.tilesPerImg * nAxes;
+    if (work > 0xffffffffull / 2) return hipErrorInvalidValue;
+    a.totalWork = (unsigned)work;
+    a.perXcd = (a.totalWork + 7) / 8;
+    const dim3 grid(a.perXcd * 8), block(kThreads);
+    const bool w2 = a.convW == 2;
+    switch (dtype) {
+    case DT_F32:
+        if (w2) hipLaunchKernelGGL((fp_kernel<DT_F32, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fp_kernel<DT_F32, false>), grid, block, 0, s, a);
+        break;
+    case DT_BF16:
+        if (w2) hipLaunchKernelGGL((fp_kernel<DT_BF16, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fp_kernel<DT_BF16, false>), grid, block, 0, s, a);
+        break;
+    case DT_F16:
+        if (w2) hipLaunchKernelGGL((fp_kernel<DT_F16, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fp_kernel<DT_F16, false>), grid, block, 0, s, a);
+        break;
+    default:
+        return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const float *axes,
+                                       int nAxes, const float *casImgs, hipStream_t s)
+{
+    if (nAxes <= 0) return hipSuccess;
+    FPArgs a;
+    a.vol = nullptr; a.axes = axes; a.out = nullptr; a.ker = g.ker;
+    a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
+    a.maskR = g.maskRadius;
+    proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
+    a.tilesPerRow = (g.M + 15) / 16;
+    a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
+    const unsigned long long work = (unsigned long long)a.tilesPerImg * nAxes;
+    if (work > 0xffffffffull / 2) return hipErrorInvalidValue;
+    a.totalWork = (unsigned)work;
+    a.perXcd = (a.totalWork + 7) / 8;
+    hipLaunchKernelGGL(adj_kernel, dim3(a.perXcd * 8), dim3(kThreads), 0, s, a, casImgs, volAcc);
+    return hipGetLastError();
+}
+
+hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
+                               const float *axes, int nAxes, const float *casImgs, hipStream_t s)
+{
+    if (nAxes <= 0) return hipSuccess;
+    BPArgs a;
+    a.vol = volAcc; a.dens = densAcc; a.axes = axes; a.img = casImgs; a.ker = g.ker;
+    a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
+    a.maskR = g.maskRadius; a.hw = g.kerHWidth;
+    proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
+    a.bricksX = (g.V + BPX - 1) / BPX;
+    const int bricksY = (g.V + BPY - 1) / BPY;
+    const int bricksZ = (g.V + BPZ - 1) / BPZ;
+    a.bricksXY = a.bricksX * bricksY;
+    a.totalWork = (unsigned)a.bricksXY * (unsigned)bricksZ;
+    a.perXcd = (a.totalWork + 7) / 8;
+    const dim3 grid(a.perXcd * 8), block(kThreads);
+    if (densAcc) hipLaunchKernelGGL(bp_kernel<true>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(bp_kernel<false>, grid, block, 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_cas_to_herm2d(const float *cas, float2 *herm, int M, int n, hipStream_t s)
+{
+    const size_t total = (size_t)n * M * (M / 2 + 1);
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_cas_to_herm2d, dim3(grid_for(total)), dim3(kThreads), 0, s, cas, herm, M, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_real2d_to_imgs(const float *real, float *imgs, int M, int N, int n, float scale,
+                                 hipStream_t s)
+{
+    const size_t total = (size_t)n * N * N;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_real2d_to_imgs, dim3(grid_for(total)), dim3(kThreads), 0, s, real, imgs, M, N,
+                       scale, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_imgs_to_real2d(const float *imgs, float *real, int M, int N, int n, hipStream_t s)
+{
+    const size_t total = (size_t)n * M * M;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_imgs_to_real2d, dim3(grid_for(total)), dim3(kThreads), 0, s, imgs, real, M, N,
+                       total);
+    return hipGetLastError();
+}
+
+hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hipStream_t s)
+{
+    const size_t total = (size_t)n * M * M;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_herm2d_to_cas, dim3(grid_for(total)), dim3(kThreads), 0, s, herm, cas, M, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_vol_to_real3d(const float *vol, int N, float *real, int M, const float *precomp,
+                                hipStream_t s)
+{
+    const size_t total = (size_t)M * M * M;
+    hipLaunchKernelGGL(k_vol_to_real3d, dim3(grid_for(total)), dim3(kThreads), 0, s, vol, N, real, M,
+                       precomp, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_herm3d_to_casvol(const float2 *herm, int M, float *casVol, int V, int pad,
+                                   hipStream_t s)
+{
+    const size_t total = (size_t)V * V * V;
+    const float gsign = ((3 * (M / 2)) & 1) ? -1.f : 1.f;
+    hipLaunchKernelGGL(k_herm3d_to_casvol, dim3(grid_for(total)), dim3(kThreads), 0, s, herm, M, casVol,
+                       V, pad, gsign, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_casvol_to_herm3d(const float *casVol, int V, int pad, float2 *herm, int M,
+                                   int zeroNyquist, hipStream_t s)
+{
+    const size_t total = (size_t)M * M * (M / 2 + 1);
+    hipLaunchKernelGGL(k_casvol_to_herm3d, dim3(grid_for(total)), dim3(kThreads), 0, s, casVol, V, pad,
+                       herm, M, zeroNyquist, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_real3d_to_vol(const float *real, int M, float *vol, int N, const float *precomp,
+                                float scale, hipStream_t s)
+{
+    const size_t total = (size_t)N * N * N;
+    const double gsign = ((3 * (M / 2)) & 1) ? -1.0 : 1.0;
+    hipLaunchKernelGGL(k_real3d_to_vol, dim3(grid_for(total)), dim3(kThreads), 0, s, real, M, vol, N,
+                       precomp, gsign * (double)scale, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce_partials(float *out, const float *parts, int nParts, size_t n,
+                                  size_t stride, int accumulate, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_reduce_partials, dim3(grid_for(n)), dim3(kThreads), 0, s, out, parts, nParts, n,
+                       stride, accumulate);
+    return hipGetLastError();
+}
+
+hipError_t launch_divide_density(float *out, const float *num, const float *den, size_t n,
+                                 double eps, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_divide_density, dim3(grid_for(n)), dim3(kThreads), 0, s, out, num, den, n, eps);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert_volume(void *dst, int dtype, const float *src, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    if (dtype == DT_BF16)
+        hipLaunchKernelGGL(k_to_bf16, dim3(grid_for(n)), dim3(kThreads), 0, s, (uint16_t *)dst, src, n);
+    else if (dtype == DT_F16)
+        hipLaunchKernelGGL(k_to_f16, dim3(grid_for(n)), dim3(kThreads), 0, s, (uint16_t *)dst, src, n);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_fill(float *p, float value, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_fill, dim3(grid_for(n)), dim3(kThreads), 0, s, p, value, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_cas_to_complex(const float *in, float2 *out, int size, int nSlices,
+                                        hipStream_t s)
+{
+    const int volumeMode = nSlices <= 0;
+    const int slices = volumeMode ? size : nSlices;
+    const size_t total = (size_t)size * size * slices;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_f_cas_to_complex, dim3(grid_for(total)), dim3(kThreads), 0, s, in, out, size,
+                       slices, volumeMode, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_complex_to_cas(const float2 *in, float *out, int size, int nSlices,
+                                        hipStream_t s)
+{
+    const size_t n = (size_t)size * size * nSlices;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_f_complex_to_cas, dim3(grid_for(n)), dim3(kThreads), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_complex_to_real(const float2 *in, float *out, int size, int nSlices,
+                                         hipStream_t s)
+{
+    const size_t n = (size_t)size * size * nSlices;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_f_complex_to_real, dim3(grid_for(n)), dim3(kThreads), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_real_to_complex(const float *in, float2 *out, int size, int nSlices,
+                                         hipStream_t s)
+{
+    const size_t n = (size_t)size * size * nSlices;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_f_real_to_complex, dim3(grid_for(n)), dim3(kThreads), 0, s, in, out, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_fftshift2d(void *data, int elemBytes, int size, int nSlices, hipStream_t s)
+{
+    if (size % 2) return hipErrorInvalidValue;
+    const size_t total = (size_t)size * (size / 2) * nSlices;
+    if (!total) return hipSuccess;
+    if (elemBytes == 4)
+        hipLaunchKernelGGL(k_f_fftshift2d<float>, dim3(grid_for(total)), dim3(kThreads), 0, s,
+                           (float *)data, size, total);
+    else if (elemBytes == 8)
+        hipLaunchKernelGGL(k_f_fftshift2d<float2>, dim3(grid_for(total)), dim3(kThreads), 0, s,
+                           (float2 *)data, size, total);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_fftshift3d(void *data, int elemBytes, int size, hipStream_t s)
+{
+    if (size % 2) return hipErrorInvalidValue;
+    const size_t total = (size_t)size * size * (size / 2);
+    if (!total) return hipSuccess;
+    if (elemBytes == 4)
+        hipLaunchKernelGGL(k_f_fftshift3d<float>, dim3(grid_for(total)), dim3(kThreads), 0, s,
+                           (float *)data, size, total);
+    else if (elemBytes == 8)
+        hipLaunchKernelGGL(k_f_fftshift3d<float2>, dim3(grid_for(total)), dim3(kThreads), 0, s,
+                           (float2 *)data, size, total);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_pad(const float *in, float *out, int inSize, int outSize, int nSlices,
+                             hipStream_t s)
+{
+    if (outSize < inSize) return hipErrorInvalidValue;
+    const size_t total = (size_t)outSize * outSize * nSlices;
+    if (!total) return hipSuccess;
+    hipLaunchKernelGGL(k_f_pad, dim3(grid_for(total)), dim3(kThreads), 0, s, in, out, inSize, outSize,
+                       total);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_crop(const void *in, void *out, int elemBytes, int inSize, int outSize,
+                              int nSlices, hipStream_t s)
+{
+    if (outSize > inSize) return hipErrorInvalidValue;
+    const size_t total = (size_t)outSize * outSize * nSlices;
+    if (!total) return hipSuccess;
+    if (elemBytes == 4)
+        hipLaunchKernelGGL(k_f_crop<float>, dim3(grid_for(total)), dim3(kThreads), 0, s,
+                           (const float *)in, (float *)out, inSize, outSize, total);
+    else if (elemBytes == 8)
+        hipLaunchKernelGGL(k_f_crop<float2>, dim3(grid_for(total)), dim3(kThreads), 0, s,
+                           (const float2 *)in, (float2 *)out, inSize, outSize, total);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_divide_scalar(float *data, float scalar, size_t n, hipStream_t s)
+{
+    if (scalar == 0.f) return hipErrorInvalidValue;
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_f_divide_scalar, dim3(grid_for(n)), dim3(kThreads), 0, s, data, scalar, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_divide_volume(float *a, const float *b, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_f_divide_volume, dim3(grid_for(n)), dim3(kThreads), 0, s, a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_multiply_volume(float *a, const float *b, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_f_multiply_volume, dim3(grid_for(n)), dim3(kThreads), 0, s, a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_multiply_complex_volume(float2 *a, const float *b, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_f_multiply_complex_volume, dim3(grid_for(n)), dim3(kThreads), 0, s, a, b, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_filter_add_volume(float *a, const float *b, size_t n, hipStream_t s)
+{
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_f_add_volume, dim3(grid_for(n)), dim3(kThreads), 0, s, a, b, n);
+    return hipGetLastError();
+}
+
+}  // namespace mgg

@@ -1,0 +1,217 @@
+"""Device-pointer operators on torch tensors (layer 2 of include/mgg.h).
+
+torch supplies device memory and streams only; every computation is a kernel
+of libMultiGPUGridder.so launched on torch's current stream.  Tensors must be
+contiguous and on the current CUDA (HIP) device.
+
+Shapes (C order, x fastest in memory as in the reference):
+  CAS volume (V, V, V); CAS images (P, M, M); images (P, N, N);
+  volume (N, N, N); directions (P, 9).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+from .hostmath import kb_table as _kb_table_host
+
+_DTYPES = {torch.float32: _lib.DTYPE_F32, torch.bfloat16: _lib.DTYPE_BF16, torch.float16: _lib.DTYPE_F16}
+_ker_cache: dict = {}
+
+
+def _stream():
+    return ctypes_void(torch.cuda.current_stream().cuda_stream)
+
+
+def ctypes_void(x):
+    import ctypes
+    return ctypes.c_void_p(int(x))
+
+
+def _ptr(t: torch.Tensor):
+    return ctypes_void(t.data_ptr())
+
+
+def _check(t: torch.Tensor, name: str, dtype=torch.float32):
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if dtype is not None and t.dtype != dtype:
+        raise ValueError(f"{name} must be {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def kb_table(device=None, hwidth: int = 2, size: int = 501) -> torch.Tensor:
+    """Device copy of the Kaiser-Bessel table (cached per device)."""
+    dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
+    key = (dev, hwidth, size)
+    t = _ker_cache.get(key)
+    if t is None:
+        t = torch.from_numpy(_kb_table_host(hwidth, size)).to(dev)
+        _ker_cache[key] = t
+    return t
+
+
+def default_mask_radius(m: int) -> float:
+    return float(np.float32(m) / 2 - 1)
+
+
+def forward_project(cas_volume: torch.Tensor, axes: torch.Tensor, m: int, mask_radius=None,
+                    ker: torch.Tensor | None = None, hwidth: float = 2.0,
+                    out: torch.Tensor | None = None) -> torch.Tensor:
+    """Reference-semantics forward projection -> (P, m, m) FP32 CAS images.
+
+    cas_volume may be float32, bfloat16 or float16 (gather in FP32)."""
+    if cas_volume.dtype not in _DTYPES:
+        raise ValueError(f"unsupported CAS volume dtype {cas_volume.dtype}")
+    _check(cas_volume, "cas_volume", None)
+    _check(axes, "axes")
+    v = cas_volume.shape[-1]
+    p = axes.numel() // 9
+    if out is None:
+        out = torch.empty((p, m, m), device=cas_volume.device, dtype=torch.float32)
+    _check(out, "out")
+    if out.numel() < p * m * m:
+        raise ValueError("out too small")
+    ker = kb_table(cas_volume.device) if ker is None else ker
+    mr = default_mask_radius(m) if mask_radius is None else float(mask_radius)
+    _lib.call("mgg_op_forward_project", _ptr(cas_volume), _DTYPES[cas_volume.dtype], v, _ptr(axes), p,
+              _ptr(out), m, mr, _ptr(ker), ker.numel(), float(hwidth), _stream())
+    return out
+
+
+def back_project(cas_images: torch.Tensor, axes: torch.Tensor, v: int, mask_radius=None,
+                 ker: torch.Tensor | None = None, hwidth: float = 2.0, out: torch.Tensor | None = None,
+                 density: torch.Tensor | None = None) -> torch.Tensor:
+    """Reference-semantics back projection, accumulated into ``out`` (V^3, zeroed if new)."""
+    _check(cas_images, "cas_images")
+    _check(axes, "axes")
+    m = cas_images.shape[-1]
+    p = axes.numel() // 9
+    if cas_images.numel() < p * m * m:
+        raise ValueError("cas_images holds fewer images than axes")
+    if out is None:
+        out = torch.zeros((v, v, v), device=cas_images.device, dtype=torch.float32)
+    _check(out, "out")
+    if density is not None:
+        _check(density, "density")
+    ker = kb_table(cas_images.device) if ker is None else ker
+    mr = default_mask_radius(m) if mask_radius is None else float(mask_radius)
+    _lib.call("mgg_op_back_project", _ptr(out), _ptr(density) if density is not None else None, v,
+              _ptr(axes), p, _ptr(cas_images), m, mr, _ptr(ker), ker.numel(), float(hwidth), _stream())
+    return out
+
+
+def back_project_adjoint(cas_images: torch.Tensor, axes: torch.Tensor, v: int, mask_radius=None,
+                         ker: torch.Tensor | None = None, hwidth: float = 2.0,
+                         out: torch.Tensor | None = None) -> torch.Tensor:
+    """Exact transpose of :func:`forward_project`, accumulated into ``out``."""
+    _check(cas_images, "cas_images")
+    _check(axes, "axes")
+    m = cas_images.shape[-1]
+    p = axes.numel() // 9
+    if out is None:
+        out = torch.zeros((v, v, v), device=cas_images.device, dtype=torch.float32)
+    _check(out, "out")
+    ker = kb_table(cas_images.device) if ker is None else ker
+    mr = default_mask_radius(m) if mask_radius is None else float(mask_radius)
+    _lib.call("mgg_op_back_project_adjoint", _ptr(out), v, _ptr(axes), p, _ptr(cas_images), m, mr,
+              _ptr(ker), ker.numel(), float(hwidth), _stream())
+    return out
+
+
+def cas_images_to_images(cas_images: torch.Tensor, n: int, out: torch.Tensor | None = None,
+                         clobber: bool = False) -> torch.Tensor:
+    """(P, m, m) CAS images -> (P, n, n) images.  The kernel uses its input as
+    scratch; unless ``clobber`` it works on a copy."""
+    _check(cas_images, "cas_images")
+    p, m = cas_images.shape[0], cas_images.shape[-1]
+    src = cas_images if clobber else cas_images.clone()
+    if out is None:
+        out = torch.empty((p, n, n), device=cas_images.device, dtype=torch.float32)
+    _check(out, "out")
+    _lib.call("mgg_op_cas_images_to_images", _ptr(src), _ptr(out), p, m, n, _stream())
+    return out
+
+
+def images_to_cas_images(images: torch.Tensor, m: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    _check(images, "images")
+    p, n = images.shape[0], images.shape[-1]
+    if out is None:
+        out = torch.empty((p, m, m), device=images.device, dtype=torch.float32)
+    _check(out, "out")
+    _lib.call("mgg_op_images_to_cas_images", _ptr(images), _ptr(out), p, m, n, _stream())
+    return out
+
+
+def volume_to_cas_volume(volume: torch.Tensor, interp_factor: float = 2.0, extra_padding: int = 3,
+                         hwidth: float = 2.0, out: torch.Tensor | None = None) -> torch.Tensor:
+    _check(volume, "volume")
+    n = volume.shape[-1]
+    m = int(np.float32(n) * np.float32(interp_factor))
+    v = m + 2 * extra_padding
+    if out is None:
+        out = torch.empty((v, v, v), device=volume.device, dtype=torch.float32)
+    _check(out, "out")
+    _lib.call("mgg_op_volume_to_cas_volume", _ptr(volume), n, _ptr(out), m, extra_padding, float(hwidth),
+              _stream())
+    return out
+
+
+def cas_volume_to_volume(cas_volume: torch.Tensor, n: int, interp_factor: float = 2.0,
+                         extra_padding: int = 3, hwidth: float = 2.0, zero_nyquist: bool = True,
+                         out: torch.Tensor | None = None) -> torch.Tensor:
+    _check(cas_volume, "cas_volume")
+    m = int(np.float32(n) * np.float32(interp_factor))
+    if out is None:
+        out = torch.empty((n, n, n), device=cas_volume.device, dtype=torch.float32)
+    _check(out, "out")
+    _lib.call("mgg_op_cas_volume_to_volume", _ptr(cas_volume), m, extra_padding, _ptr(out), n,
+              float(hwidth), int(bool(zero_nyquist)), _stream())
+    return out
+
+
+def divide_density(num: torch.Tensor, den: torch.Tensor, eps: float = 1e-6,
+                   out: torch.Tensor | None = None) -> torch.Tensor:
+    _check(num, "num")
+    _check(den, "den")
+    out = torch.empty_like(num) if out is None else out
+    _lib.call("mgg_op_divide_density", _ptr(out), _ptr(num), _ptr(den), num.numel(), float(eps), _stream())
+    return out
+
+
+def reduce_partials(parts: torch.Tensor, out: torch.Tensor | None = None, accumulate: bool = False):
+    """Sum (K, ...) partial volumes along dim 0 in FP64 -> FP32."""
+    _check(parts, "parts")
+    k = parts.shape[0]
+    n = parts[0].numel()
+    if out is None:
+        out = torch.empty(parts.shape[1:], device=parts.device, dtype=torch.float32)
+    _check(out, "out")
+    _lib.call("mgg_op_reduce_partials", _ptr(out), _ptr(parts), k, n, n, int(accumulate), _stream())
+    return out
+
+
+def convert_volume(cas_volume: torch.Tensor, dtype: torch.dtype) -> torch.Tensor:
+    """FP32 CAS volume -> BF16 / FP16 storage copy (round to nearest even)."""
+    _check(cas_volume, "cas_volume")
+    if dtype not in (torch.bfloat16, torch.float16):
+        raise ValueError("dtype must be bfloat16 or float16")
+    out = torch.empty(cas_volume.shape, device=cas_volume.device, dtype=dtype)
+    _lib.call("mgg_op_convert_volume", _ptr(out), _DTYPES[dtype], _ptr(cas_volume), cas_volume.numel(), _stream())
+    return out
+
+
+# -- standalone filters (Filters/*.cu semantics) ----------------------------------
+def filter_call(name: str, *args):
+    """Call an mgg_filter_* entry point; tensors are passed as device pointers."""
+    conv = []
+    for a in args:
+        if isinstance(a, torch.Tensor):
+            if not a.is_cuda or not a.is_contiguous():
+                raise ValueError("filter tensors must be contiguous device tensors")
+            conv.append(_ptr(a))
+        else:
+            conv.append(a)
+    _lib.call(name, *conv, _stream())

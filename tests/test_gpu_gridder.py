@@ -1,0 +1,149 @@
+"""GPU tests of the gridder object (the drop-in boundary): the Matlab-class
+mirror and the legacy ctypes symbols, end to end against the oracle."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _oracle_forward(oracle, ker, vol, axes, n):
+    cas = oracle.vol_to_cas(vol, 2, 3).astype(np.float32)
+    return cas, oracle.cas_imgs_to_imgs(oracle.forward_project(cas, axes, ker, 2 * n), n)
+
+
+@pytest.mark.parametrize("run_fft_on_gpu", [1, 0])
+@pytest.mark.parametrize("n,p,streams", [(16, 25, 4), (32, 101, 10), (32, 7, 10)])
+def test_forward_project_object(cuda, oracle, ker32, run_fft_on_gpu, n, p, streams):
+    from multigpugridder_amd import MultiGPUGridder, blob_volume, random_directions
+
+    print(f"object FP n={n} p={p} fft_on_gpu={run_fft_on_gpu}", flush=True)
+    vol = blob_volume(n, 7)
+    axes = random_directions(p, 17)
+    g = MultiGPUGridder(n, p, 2, RunFFTOnGPU=run_fft_on_gpu, nStreamsFP=streams, MaxAxesToAllocate=40)
+    g.setVolume(vol)
+    imgs = g.forwardProject(axes)
+    cas, ref = _oracle_forward(oracle, ker32, vol, axes, n)
+    assert imgs.shape == (p, n, n)
+    assert rel_l2(imgs, ref) < TOL
+    # the device CAS volume went back to the caller's array
+    assert rel_l2(g.CASVolume, oracle.vol_to_cas(vol, 2, 3)) < TOL
+    g.delete()
+
+
+@pytest.mark.parametrize("run_fft_on_gpu", [1, 0])
+def test_back_project_object(cuda, oracle, ker32, run_fft_on_gpu):
+    from multigpugridder_amd import MultiGPUGridder, random_directions
+
+    n, p = 16, 33
+    m, v = 2 * n, 2 * n + 6
+    axes = random_directions(p, 5)
+    imgs = np.random.default_rng(2).standard_normal((p, n, n)).astype(np.float32)
+    g = MultiGPUGridder(n, p, 2, RunFFTOnGPU=run_fft_on_gpu, nStreamsBP=3, MaxAxesToAllocate=10)
+    g.backProject(imgs, axes)
+    cas_vol = g.casVolumeToHost()
+    cas_imgs = oracle.imgs_to_cas_imgs(imgs, m).astype(np.float32)
+    ref = oracle.back_project(cas_imgs, axes, ker32, v)
+    assert rel_l2(cas_vol, ref) < TOL
+    # BP zeroes the device volume first (gpuGridder.cpp:330-333): a second call gives the same result
+    g.backProject(imgs, axes)
+    assert rel_l2(g.casVolumeToHost(), ref) < TOL
+    vol = g.getVol()
+    assert rel_l2(vol, oracle.cas_to_vol(ref, n)) < TOL
+    g.delete()
+
+
+def test_reconstruct_object(cuda, oracle, ker32):
+    from multigpugridder_amd import MultiGPUGridder, blob_volume
+
+    n = 16
+    m, v = 2 * n, 2 * n + 6
+    axes = oracle.uniform_axes(6, 12)
+    p = axes.shape[0]
+    vol = blob_volume(n, 4)
+    g = MultiGPUGridder(n, p, 2)
+    g.setVolume(vol)
+    imgs = g.forwardProject(axes)
+    rec = g.reconstructVol(imgs, axes)
+    cas_imgs = oracle.imgs_to_cas_imgs(imgs, m).astype(np.float32)
+    bp = oracle.back_project(cas_imgs, axes, ker32, v)
+    dens = oracle.back_project(np.ones_like(cas_imgs), axes, ker32, v)
+    ref = oracle.reconstruct(bp, dens, n)
+    assert rel_l2(rec, ref) < 1e-4  # the 1e-6 in the density division amplifies FP32 noise where density ~ 0
+    # the reference's own acceptance check (BackProjectTests.m:57-66): max |recon - vol| <= 1
+    assert np.abs(rec - vol).max() <= 1.0 + np.abs(vol).max()
+    g.delete()
+
+
+def test_reset_and_several_gridders_share_a_gpu(cuda, oracle, ker32):
+    from multigpugridder_amd import MultiGPUGridder, blob_volume, random_directions
+
+    a = MultiGPUGridder(16, 10, 2)
+    b = MultiGPUGridder(24, 10, 2)
+    axes = random_directions(10, 1)
+    va, vb = blob_volume(16, 1), blob_volume(24, 2)
+    a.setVolume(va)
+    b.setVolume(vb)
+    ia = a.forwardProject(axes)
+    ib = b.forwardProject(axes)
+    assert rel_l2(ia, _oracle_forward(oracle, ker32, va, axes, 16)[1]) < TOL
+    assert rel_l2(ib, _oracle_forward(oracle, ker32, vb, axes, 24)[1]) < TOL
+    a.resetVolume()
+    assert not a.Volume.any() and not a.CASVolume.any()
+    a.delete()
+    b.delete()
+
+
+def test_errors_are_status_codes(cuda):
+    from multigpugridder_amd import MggError, MultiGPUGridder, _lib
+
+    g = MultiGPUGridder(16, 4, 2)
+    g._numAxes = 0
+    _lib.call("mgg_set_num_axes", g.objectHandle, 0)
+    with pytest.raises(MggError) as e:
+        _lib.call("mgg_forward_project", g.objectHandle)
+    assert e.value.code == _lib.MGG_ERR_INVALID_ARGUMENT
+    with pytest.raises(MggError) as e:
+        _lib.call("mgg_set_num_axes", g.objectHandle, 100)
+        _lib.call("mgg_forward_project", g.objectHandle)
+    assert e.value.code == _lib.MGG_ERR_INVALID_ARGUMENT  # more axes than the array holds
+    h = ctypes.c_void_p()
+    assert _lib.lib().mgg_create(16, 4, 2.0, 3, 1, _lib.int_array([99]), 1, 0, ctypes.byref(h)) == _lib.MGG_ERR_DEVICE
+    g.delete()
+
+
+def test_legacy_ctypes_symbols(cuda, oracle, ker32):
+    """Python/MultiGPUGridder_Python.py flow: CAS volume in, CAS images out,
+    CAS images back into the CAS volume."""
+    from multigpugridder_amd import _lib
+    from multigpugridder_amd.synth import random_directions
+
+    lib = _lib.lib()
+    v = m = 32
+    cas = oracle.vol_to_cas(np.random.default_rng(0).random((10, 10, 10)), 2.6, 3)[:v, :v, :v]
+    cas = np.ascontiguousarray(cas, dtype=np.float32)
+    axes = random_directions(6, 3)
+    h = lib.Gridder_new()
+    assert h
+    assert lib.SetNumberGPUs(h, 1) and lib.SetNumberStreams(h, 4)
+    fp = ctypes.POINTER(ctypes.c_float)
+    assert lib.SetAxes(h, axes.ctypes.data_as(fp), _lib.int_array([axes.size, 1, 1]))
+    assert lib.SetVolume(h, cas.ctypes.data_as(fp), _lib.int_array([v, v, v]))
+    assert lib.SetImgSize(h, _lib.int_array([m, m, m]))
+    r = ctypes.c_float(m / 2 - 2)
+    assert lib.SetMaskRadius(h, ctypes.byref(r))
+    assert lib.Forward_Project(h)
+    out = np.ctypeslib.as_array(lib.GetImages(h), shape=(6, m, m)).copy()
+    ref = oracle.forward_project(cas, axes, ker32, m, mask_radius=m / 2 - 2)
+    assert rel_l2(out, ref) < TOL
+    assert lib.ResetVolume(h)
+    assert lib.SetImages(h, out.ctypes.data_as(fp))
+    assert lib.Back_Project(h)
+    vol = np.ctypeslib.as_array(lib.GetVolume(h), shape=(v, v, v))
+    assert rel_l2(vol, oracle.back_project(out, axes, ker32, v, mask_radius=m / 2 - 2)) < TOL
+    lib.mgg_destroy(h)
