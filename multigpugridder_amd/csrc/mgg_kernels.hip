@@ -283,6 +283,7 @@ struct BPArgs {
     int V, M, K, nAxes;
     float maskR, kerScale, kerCenter, fW, hw;
     int convW, bricksX, bricksXY;
+    int maskD2;  // largest integer d2 with sqrtf((float)d2) <= maskR (-1: none)
     unsigned perXcd, totalWork;
 };
 
@@ -422,6 +423,241 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BPArgs p)
         const size_t vidx = ((size_t)vk * V + vj) * V + vi;
         p.vol[vidx] = (float)((double)p.vol[vidx] + accAll);
         if (DENS) p.dens[vidx] = (float)((double)p.dens[vidx] + densAll);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Back projection fast path (kerHWidth 2, 501-entry table: kerScale 125,
+// kerCenter 250), same reference semantics as bp_kernel.
+//
+// Work layout.  A workgroup owns an 8x8x8 brick and the list of directions
+// whose slab can touch it (culled and compacted in direction order, as in
+// bp_kernel).  Wave w takes list entries w, w+4, w+8, ...  For each direction
+// the wave first finds the brick voxels inside the slab |fk| <= 2 (lane l
+// tests the 8 voxels of column (l & 7, l >> 3)), compacts them into a
+// per-wave queue in LDS, and then runs the 25-tap gather on 64 queued voxels
+// at a time, so every lane carries an active voxel (a fixed voxel-per-lane
+// layout leaves most lanes idle: a slab activates ~1/3 of a brick).
+// Contributions go to a per-wave LDS accumulator; one direction never queues
+// a voxel twice, so lanes never collide, and the four wave accumulators are
+// combined in a fixed order at the end: the result is deterministic.
+//
+// Table index.  For every FP32 r in [-2, 2],
+// floor(fl(fl(r*125) + 250) + 0.5) equals the reference's
+// roundf(fl(fl(r*125) + 250)) (checked exhaustively), and
+// v_cvt_rpi_i32_f32 computes floor(x + 0.5) in one instruction.  The LDS table
+// is padded on both sides with copies of its end entries, so offsets beyond
+// +-2 land on the entry the reference's clamp would pick and r needs no clamp.
+//
+// Mask.  sqrtf(d2) <= maskRadius with d2 a small integer is monotone in d2,
+// so the host passes the largest passing d2 and the test is an integer
+// compare.
+// ---------------------------------------------------------------------------
+constexpr int BB = 8;                  // brick edge
+constexpr int BV = BB * BB * BB;       // voxels per brick
+constexpr int KPAD = 320;              // padding entries on each side of the table
+constexpr int KFAST = 501;
+constexpr int KTOT = KFAST + 2 * KPAD; // padded table length
+
+__device__ __forceinline__ int cvt_rpi(float x)
+{
+    int i;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    return i;
+}
+
+__device__ __forceinline__ float kb_fast(const float *__restrict__ sk, float r)
+{
+    float s = r * 125.0f;
+    s = s + 250.0f;
+    int i = cvt_rpi(s);  // floor(s + 0.5)
+    i = min(max(i, -KPAD), KFAST - 1 + KPAD);
+    return sk[KPAD + i];
+}
+
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+template <bool DENS>
+__global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
+{
+    __shared__ float sker[KTOT];
+    __shared__ float sax[kThreads * 9];
+    __shared__ int sidx[kThreads];
+    __shared__ int swave[kThreads / 64];
+    __shared__ float sacc[4][BV];
+    __shared__ float sden[DENS ? 4 : 1][DENS ? BV : 1];
+    __shared__ unsigned short squeue[4][BV];
+
+    const unsigned work = xcd_work(p.perXcd);
+    if (work >= p.totalWork) return;
+    const int bz = (int)(work / (unsigned)p.bricksXY);
+    const int rem = (int)(work - (unsigned)bz * (unsigned)p.bricksXY);
+    const int by = rem / p.bricksX, bx = rem - by * p.bricksX;
+    const int ox = bx * BB, oy = by * BB, oz = bz * BB;
+    const int V = p.V, M = p.M, cv = V / 2, ci = M / 2;
+    {
+        float dx = fmaxf(fmaxf((float)(ox - cv), (float)(cv - (ox + BB - 1))), 0.f);
+        float dy = fmaxf(fmaxf((float)(oy - cv), (float)(cv - (oy + BB - 1))), 0.f);
+        float dz = fmaxf(fmaxf((float)(oz - cv), (float)(cv - (oz + BB - 1))), 0.f);
+        if (dx * dx + dy * dy + dz * dz > (p.maskR + 1.f) * (p.maskR + 1.f)) return;
+    }
+    const int t = threadIdx.x;
+    for (int k = t; k < KTOT; k += kThreads) sker[k] = p.ker[min(max(k - KPAD, 0), KFAST - 1)];
+    for (int k = t; k < 4 * BV; k += kThreads) {
+        (&sacc[0][0])[k] = 0.f;
+        if (DENS) (&sden[0][0])[k] = 0.f;
+    }
+
+    const float cvf = (float)cv, cif = (float)ci;
+    const float bc = 0.5f * (BB - 1);
+    const float bcx = (float)ox + bc - cvf, bcy = (float)oy + bc - cvf, bcz = (float)oz + bc - cvf;
+    const int M2 = M * M;
+    const int wv = t >> 6, ln = t & 63;
+    // This lane's voxel column for the slab test: (ox + cx, oy + cy, oz + 0..7).
+    const int cx = ln & 7, cy = ln >> 3;
+    const int vi0 = ox + cx, vj0 = oy + cy;
+    unsigned vmask = 0;  // inside the volume and the mask sphere (direction-independent)
+#pragma unroll
+    for (int k = 0; k < BB; k++) {
+        const int vk = oz + k;
+        const int d2 = (vi0 - cv) * (vi0 - cv) + (vj0 - cv) * (vj0 - cv) + (vk - cv) * (vk - cv);
+        if (vi0 < V && vj0 < V && vk < V && d2 <= p.maskD2) vmask |= 1u << k;
+    }
+    const float dvi0 = (float)vi0 - cv, dvj0 = (float)vj0 - cv;
+
+    for (int base = 0; base < p.nAxes; base += kThreads) {
+        const int n = min(kThreads, p.nAxes - base);
+        __syncthreads();
+        bool pass = false;
+        float a9[9];
+        if (t < n) {
+            const float *ax = p.axes + (size_t)9 * (base + t);
+#pragma unroll
+            for (int k = 0; k < 9; k++) a9[k] = ax[k];
+            const float d = bcx * a9[6] + bcy * a9[7] + bcz * a9[8];
+            const float sup = bc * (fabsf(a9[6]) + fabsf(a9[7]) + fabsf(a9[8]));
+            pass = fabsf(d) <= p.hw + sup + 1e-3f;
+        }
+        const unsigned long long bal = __ballot(pass);
+        if (ln == 0) swave[wv] = __popcll(bal);
+        __syncthreads();
+        int off = 0, total = 0;
+#pragma unroll
+        for (int w = 0; w < kThreads / 64; w++) {
+            off += (w < wv) ? swave[w] : 0;
+            total += swave[w];
+        }
+        if (pass) {
+            const int pos = off + __popcll(bal & ((1ull << ln) - 1ull));
+            sidx[pos] = base + t;
+#pragma unroll
+            for (int k = 0; k < 9; k++) sax[pos * 9 + k] = a9[k];
+        }
+        __syncthreads();
+
+        for (int q = wv; q < total; q += 4) {
+            const float *ax = sax + q * 9;
+            const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
+            const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
+            const float n0 = ax[6], n1 = ax[7], n2 = ax[8];
+            // Slab test of the lane's column (pinned FP32 order) and
+            // compaction into the wave's queue, ordered by (z, lane).
+            const float fk0 = dvi0 * n0 + dvj0 * n1;
+            wave_lds_sync();  // previous direction's queue fully read
+            int nq = 0;
+#pragma unroll
+            for (int k = 0; k < BB; k++) {
+                const float dvk = (float)(oz + k) - cv;
+                const float fk = fk0 + dvk * n2;
+                const bool on = ((vmask >> k) & 1u) && fabsf(fk) <= p.hw;
+                const unsigned long long b = __ballot(on);
+                if (on) squeue[wv][nq + __popcll(b & ((1ull << ln) - 1ull))] =
+                            (unsigned short)(cx + BB * (cy + BB * k));
+                nq += __popcll(b);
+            }
+            wave_lds_sync();
+            if (nq == 0) continue;
+            const float *im = p.img + (size_t)sidx[q] * M2;
+            for (int e0 = 0; e0 < nq; e0 += 64) {
+                const int e = e0 + ln;
+                if (e >= nq) break;
+                const int vox = squeue[wv][e];
+                const int lx = vox & 7, ly = (vox >> 3) & 7, lz = vox >> 6;
+                const int vi = ox + lx, vj = oy + ly, vk = oz + lz;
+                const float dvi = (float)vi - cv, dvj = (float)vj - cv, dvk = (float)vk - cv;
+                const float fi = dvi * u0 + dvj * u1 + dvk * u2 + cif;
+                const float fj = dvi * v0 + dvj * v1 + dvk * v2 + cif;
+                const int I = (int)roundf(fi), J = (int)roundf(fj);
+                const float fvi = (float)vi, fvj = (float)vj, fvk = (float)vk;
+                float Ax[5], Ay[5], Az[5], Bx[5], By[5], Bz[5];
+#pragma unroll
+                for (int s5 = 0; s5 < 5; s5++) {
+                    const float di = (float)(I - 2 + s5 - ci), dj = (float)(J - 2 + s5 - ci);
+                    Ax[s5] = di * u0; Ay[s5] = di * u1; Az[s5] = di * u2;
+                    Bx[s5] = dj * v0; By[s5] = dj * v1; Bz[s5] = dj * v2;
+                }
+                float cum = 0.f, dsum = 0.f;
+                const bool inside = I >= 2 && I + 2 < M && J >= 2 && J + 2 < M;
+#pragma unroll
+                for (int jj = 0; jj < 5; jj++) {
+                    float val[5];
+                    const int row = (J - 2 + jj) * M + (I - 2);
+                    if (inside) {
+                        f4 a;
+                        __builtin_memcpy(&a, im + row, 16);
+                        val[0] = a.x; val[1] = a.y; val[2] = a.z; val[3] = a.w;
+                        val[4] = im[row + 4];
+                    } else {
+                        // Reference bound on the linear index only
+                        // (gpuBackProject.cu:99): wraps across rows.
+#pragma unroll
+                        for (int ii = 0; ii < 5; ii++) {
+                            const int lin = row + ii;
+                            val[ii] = (lin >= 0 && lin < M2) ? im[lin] : 0.f;
+                        }
+                    }
+#pragma unroll
+                    for (int ii = 0; ii < 5; ii++) {
+                        const float px = Ax[ii] + Bx[jj] + cvf;
+                        const float py = Ay[ii] + By[jj] + cvf;
+                        const float pz = Az[ii] + Bz[jj] + cvf;
+                        float w = kb_fast(sker, px - fvi);
+                        w = w * kb_fast(sker, py - fvj);
+                        w = w * kb_fast(sker, pz - fvk);
+                        cum = fmaf(val[ii], w, cum);
+                        if (DENS) {
+                            const int lin = row + ii;
+                            dsum += (inside || (lin >= 0 && lin < M2)) ? w : 0.f;
+                        }
+                    }
+                }
+                sacc[wv][vox] += cum;
+                if (DENS) sden[wv][vox] += dsum;
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = t; k < BV; k += kThreads) {
+        const int lx = k & 7, ly = (k >> 3) & 7, lz = k >> 6;
+        const int vi = ox + lx, vj = oy + ly, vk = oz + lz;
+        if (vi >= V || vj >= V || vk >= V) continue;
+        const int d2 = (vi - cv) * (vi - cv) + (vj - cv) * (vj - cv) + (vk - cv) * (vk - cv);
+        if (d2 > p.maskD2) continue;
+        const size_t vidx = ((size_t)vk * V + vj) * V + vi;
+        double s = (double)p.vol[vidx];
+#pragma unroll
+        for (int w = 0; w < 4; w++) s += (double)sacc[w][k];
+        p.vol[vidx] = (float)s;
+        if (DENS) {
+            double d = (double)p.dens[vidx];
+#pragma unroll
+            for (int w = 0; w < 4; w++) d += (double)sden[w][k];
+            p.dens[vidx] = (float)d;
+        }
     }
 }
 
@@ -833,6 +1069,22 @@ inline unsigned grid_for(size_t n)
     return (unsigned)g;
 }
 
+// Largest integer d2 (a squared voxel distance, at most 3*(v/2)^2) with
+// sqrtf((float)d2) <= r, i.e. the integer form of the reference's mask test
+// sqrtf(...) <= maskRadius (gpuBackProject.cu:48-50).  -1 if none passes.
+inline int mask_d2(float r, int v)
+{
+    const long long hi = 3LL * (v / 2 + 1) * (v / 2 + 1);
+    if (!(sqrtf(0.f) <= r)) return -1;
+    long long lo = 0, top = hi;  // sqrtf is monotone: binary search the last passing d2
+    if (sqrtf((float)top) <= r) return (int)top;
+    while (top - lo > 1) {
+        const long long mid = (lo + top) / 2;
+        if (sqrtf((float)mid) <= r) lo = mid; else top = mid;
+    }
+    return (int)lo;
+}
+
 inline void proj_consts(const ProjGeom &g, float &kerScale, float &kerCenter, float &fW,
                         int &convW)
 {
@@ -912,15 +1164,24 @@ hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
     a.maskR = g.maskRadius; a.hw = g.kerHWidth;
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
-    a.bricksX = (g.V + BPX - 1) / BPX;
-    const int bricksY = (g.V + BPY - 1) / BPY;
-    const int bricksZ = (g.V + BPZ - 1) / BPZ;
+    const bool fast = a.convW == 2 && a.kerScale == 125.0f && a.kerCenter == 250.0f && g.K == KFAST &&
+                      g.V <= 8192;
+    a.maskD2 = mask_d2(g.maskRadius, g.V);
+    const int ex = fast ? BB : BPX, ey = fast ? BB : BPY, ez = fast ? BB : BPZ;
+    a.bricksX = (g.V + ex - 1) / ex;
+    const int bricksY = (g.V + ey - 1) / ey;
+    const int bricksZ = (g.V + ez - 1) / ez;
     a.bricksXY = a.bricksX * bricksY;
     a.totalWork = (unsigned)a.bricksXY * (unsigned)bricksZ;
     a.perXcd = (a.totalWork + 7) / 8;
     const dim3 grid(a.perXcd * 8), block(kThreads);
-    if (densAcc) hipLaunchKernelGGL(bp_kernel<true>, grid, block, 0, s, a);
-    else hipLaunchKernelGGL(bp_kernel<false>, grid, block, 0, s, a);
+    if (fast) {
+        if (densAcc) hipLaunchKernelGGL(bp_fast_kernel<true>, grid, block, 0, s, a);
+        else hipLaunchKernelGGL(bp_fast_kernel<false>, grid, block, 0, s, a);
+    } else {
+        if (densAcc) hipLaunchKernelGGL(bp_kernel<true>, grid, block, 0, s, a);
+        else hipLaunchKernelGGL(bp_kernel<false>, grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 

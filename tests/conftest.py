@@ -21,6 +21,20 @@ def rel_l2(a, b):
     return float(np.linalg.norm((a - b).ravel()) / (den if den > 0 else 1.0))
 
 
+def assert_parity(got, ref, tol=1e-5, peak=2e-6):
+    """Relative L2 within ``tol`` and no element off by more than ``peak``
+    times the largest reference magnitude.  FP32 accumulation noise stays
+    ~10x below ``peak``; a single flipped rounding decision (a coordinate or
+    kernel-table index) moves an element by ~1e-4 of the peak and fails."""
+    got = np.asarray(got, dtype=np.float64)
+    ref = np.asarray(ref, dtype=np.float64)
+    err = rel_l2(got, ref)
+    scale = float(np.max(np.abs(ref))) if ref.size else 0.0
+    worst = float(np.max(np.abs(got - ref))) if ref.size else 0.0
+    assert err < tol, f"relative L2 {err:.3e} >= {tol:.1e}"
+    assert worst <= peak * scale, f"max abs error {worst:.3e} > {peak:.1e} * {scale:.3e}"
+
+
 @pytest.fixture(scope="session")
 def oracle():
     from oracle import oracle as o
