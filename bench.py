@@ -49,6 +49,19 @@ def bytes_bp(m: int) -> int:
     return 8 * 5 * n_mask(m) + 4 * m * m + 36
 
 
+def measured_traffic(path: str, n: int, kernel: str, chunk: int):
+    """HBM bytes per launch of the dominant kernel, from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summary (a line 'bytes_per_direction <kernel> N=<n>: <B>'),
+    scaled to this run's directions per launch; None when not measured."""
+    if not os.path.exists(path):
+        return None
+    key = f"bytes_per_direction {kernel} N={n}:"
+    for line in open(path):
+        if line.strip().startswith(key):
+            return int(float(line.split(":", 1)[1].split()[0]) * chunk)
+    return None
+
+
 def cpu_baseline(n: int, p_sample: int, seed: int = 1):
     """Time the oracle's FP32 OpenMP path (the reference kernels' loop
     structure) on p_sample directions of the same geometry."""
@@ -91,6 +104,11 @@ def main():
     ap.add_argument("--chunk", type=int, default=1000, help="directions per launch")
     ap.add_argument("--cpu-sample", type=int, default=400)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="also run one device-to-device copy of this many MiB (a known-size "
+                         "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
+    ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "bp_kernel_traffic_n128.md"),
+                    help="profile summary holding the measured HBM bytes per back-projection direction")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -148,6 +166,13 @@ def main():
             timed("bp_chain", lambda: ops.images_to_cas_images(imgs[s:s + c], m, out=cas_imgs[:c]), record)
             timed("bp", lambda: ops.back_project(cas_imgs[:c], ax, v, ker=ker, out=bp_vol), record)
 
+    if args.calib_copy_mb:
+        src = torch.ones(args.calib_copy_mb * (1 << 20) // 4, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
+
     for _ in range(args.warmup):
         step(False)
     torch.cuda.synchronize()
@@ -170,6 +195,7 @@ def main():
     launch_ms = bp_launch_ms if dominant == "bp" else fp_launch_ms
     achieved = per_dir * chunk / (launch_ms * 1e-3) / 1e9
     value = p / (ms_per_step * 1e-3)
+    traffic = measured_traffic(args.traffic_file, n, dominant, chunk)
 
     out = {
         "metric": METRIC,
@@ -197,7 +223,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_direction": per_dir,
             "directions_per_launch": chunk,
             "avg_launch_ms": round(launch_ms, 4),

@@ -438,9 +438,12 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BPArgs p)
 // per-wave queue in LDS, and then runs the 25-tap gather on 64 queued voxels
 // at a time, so every lane carries an active voxel (a fixed voxel-per-lane
 // layout leaves most lanes idle: a slab activates ~1/3 of a brick).
-// Contributions go to a per-wave LDS accumulator; one direction never queues
-// a voxel twice, so lanes never collide, and the four wave accumulators are
-// combined in a fixed order at the end: the result is deterministic.
+// The queue spans directions (entries carry a 7-bit direction slot), so the
+// tail of one direction fills up with the next and groups stay full.  Two
+// entries of one group can name the same voxel (different directions), so
+// contributions go to a per-wave LDS accumulator with LDS atomic adds; the
+// four wave accumulators are combined in a fixed order at the end and added
+// to the volume in FP64.
 //
 // Table index.  For every FP32 r in [-2, 2],
 // floor(fl(fl(r*125) + 250) + 0.5) equals the reference's
@@ -458,6 +461,8 @@ constexpr int BV = BB * BB * BB;       // voxels per brick
 constexpr int KPAD = 320;              // padding entries on each side of the table
 constexpr int KFAST = 501;
 constexpr int KTOT = KFAST + 2 * KPAD; // padded table length
+constexpr int QCAP = 1024;             // per-wave queue ring (>= BV + 64, power of 2)
+constexpr int BPCH = 128;              // directions culled per pass (queue entries hold 9-bit voxel + 7-bit slot)
 
 __device__ __forceinline__ int cvt_rpi(float x)
 {
@@ -485,12 +490,12 @@ template <bool DENS>
 __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
 {
     __shared__ float sker[KTOT];
-    __shared__ float sax[kThreads * 9];
-    __shared__ int sidx[kThreads];
+    __shared__ float sax[BPCH * 9];
+    __shared__ int sidx[BPCH];
     __shared__ int swave[kThreads / 64];
     __shared__ float sacc[4][BV];
     __shared__ float sden[DENS ? 4 : 1][DENS ? BV : 1];
-    __shared__ unsigned short squeue[4][BV];
+    __shared__ unsigned short squeue[4][QCAP];
 
     const unsigned work = xcd_work(p.perXcd);
     if (work >= p.totalWork) return;
@@ -529,8 +534,8 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
     }
     const float dvi0 = (float)vi0 - cv, dvj0 = (float)vj0 - cv;
 
-    for (int base = 0; base < p.nAxes; base += kThreads) {
-        const int n = min(kThreads, p.nAxes - base);
+    for (int base = 0; base < p.nAxes; base += BPCH) {
+        const int n = min(BPCH, p.nAxes - base);
         __syncthreads();
         bool pass = false;
         float a9[9];
@@ -559,85 +564,100 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
         }
         __syncthreads();
 
-        for (int q = wv; q < total; q += 4) {
-            const float *ax = sax + q * 9;
-            const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
-            const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
-            const float n0 = ax[6], n1 = ax[7], n2 = ax[8];
-            // Slab test of the lane's column (pinned FP32 order) and
-            // compaction into the wave's queue, ordered by (z, lane).
-            const float fk0 = dvi0 * n0 + dvj0 * n1;
-            wave_lds_sync();  // previous direction's queue fully read
-            int nq = 0;
+        // Directions of this chunk that wave wv handles: q = wv, wv+4, ...
+        // Their active voxels stream through the wave's FIFO queue; every
+        // full group of 64 entries is gathered with all lanes busy.
+        int head = 0, tail = 0;  // queue indices (uniform), ring of size QCAP
+        for (int q = wv; q < total || tail > head; q += 4) {
+            if (q < total) {
+                const float *ax = sax + q * 9;
+                const float n0 = ax[6], n1 = ax[7], n2 = ax[8];
+                // Slab test of the lane's column (pinned FP32 order), appended
+                // to the queue in (z, lane) order.
+                const float fk0 = dvi0 * n0 + dvj0 * n1;
 #pragma unroll
-            for (int k = 0; k < BB; k++) {
-                const float dvk = (float)(oz + k) - cv;
-                const float fk = fk0 + dvk * n2;
-                const bool on = ((vmask >> k) & 1u) && fabsf(fk) <= p.hw;
-                const unsigned long long b = __ballot(on);
-                if (on) squeue[wv][nq + __popcll(b & ((1ull << ln) - 1ull))] =
-                            (unsigned short)(cx + BB * (cy + BB * k));
-                nq += __popcll(b);
-            }
-            wave_lds_sync();
-            if (nq == 0) continue;
-            const float *im = p.img + (size_t)sidx[q] * M2;
-            for (int e0 = 0; e0 < nq; e0 += 64) {
-                const int e = e0 + ln;
-                if (e >= nq) break;
-                const int vox = squeue[wv][e];
-                const int lx = vox & 7, ly = (vox >> 3) & 7, lz = vox >> 6;
-                const int vi = ox + lx, vj = oy + ly, vk = oz + lz;
-                const float dvi = (float)vi - cv, dvj = (float)vj - cv, dvk = (float)vk - cv;
-                const float fi = dvi * u0 + dvj * u1 + dvk * u2 + cif;
-                const float fj = dvi * v0 + dvj * v1 + dvk * v2 + cif;
-                const int I = (int)roundf(fi), J = (int)roundf(fj);
-                const float fvi = (float)vi, fvj = (float)vj, fvk = (float)vk;
-                float Ax[5], Ay[5], Az[5], Bx[5], By[5], Bz[5];
-#pragma unroll
-                for (int s5 = 0; s5 < 5; s5++) {
-                    const float di = (float)(I - 2 + s5 - ci), dj = (float)(J - 2 + s5 - ci);
-                    Ax[s5] = di * u0; Ay[s5] = di * u1; Az[s5] = di * u2;
-                    Bx[s5] = dj * v0; By[s5] = dj * v1; Bz[s5] = dj * v2;
+                for (int k = 0; k < BB; k++) {
+                    const float dvk = (float)(oz + k) - cv;
+                    const float fk = fk0 + dvk * n2;
+                    const bool on = ((vmask >> k) & 1u) && fabsf(fk) <= p.hw;
+                    const unsigned long long b = __ballot(on);
+                    if (on)
+                        squeue[wv][(tail + __popcll(b & ((1ull << ln) - 1ull))) & (QCAP - 1)] =
+                            (unsigned short)((cx + BB * (cy + BB * k)) | (q << 9));
+                    tail += __popcll(b);
                 }
-                float cum = 0.f, dsum = 0.f;
-                const bool inside = I >= 2 && I + 2 < M && J >= 2 && J + 2 < M;
+            }
+            const bool last = q + 4 >= total;
+            wave_lds_sync();
+            // Gather full groups (and, after the last direction, the rest).
+            while (tail - head >= 64 || (last && tail > head)) {
+                const int e = head + ln;
+                head = min(head + 64, tail);
+                if (e < tail) {
+                    const unsigned ent = squeue[wv][e & (QCAP - 1)];
+                    const int vox = ent & 511, qq = ent >> 9;
+                    const float *ax = sax + qq * 9;
+                    const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
+                    const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
+                    const float *im = p.img + (size_t)sidx[qq] * M2;
+                    const int lx = vox & 7, ly = (vox >> 3) & 7, lz = vox >> 6;
+                    const int vi = ox + lx, vj = oy + ly, vk = oz + lz;
+                    const float dvi = (float)vi - cv, dvj = (float)vj - cv, dvk = (float)vk - cv;
+                    const float fi = dvi * u0 + dvj * u1 + dvk * u2 + cif;
+                    const float fj = dvi * v0 + dvj * v1 + dvk * v2 + cif;
+                    const int I = (int)roundf(fi), J = (int)roundf(fj);
+                    const float fvi = (float)vi, fvj = (float)vj, fvk = (float)vk;
+                    float Ax[5], Ay[5], Az[5];
 #pragma unroll
-                for (int jj = 0; jj < 5; jj++) {
-                    float val[5];
-                    const int row = (J - 2 + jj) * M + (I - 2);
-                    if (inside) {
-                        f4 a;
-                        __builtin_memcpy(&a, im + row, 16);
-                        val[0] = a.x; val[1] = a.y; val[2] = a.z; val[3] = a.w;
-                        val[4] = im[row + 4];
-                    } else {
-                        // Reference bound on the linear index only
-                        // (gpuBackProject.cu:99): wraps across rows.
+                    for (int s5 = 0; s5 < 5; s5++) {
+                        const float di = (float)(I - 2 + s5 - ci);
+                        Ax[s5] = di * u0; Ay[s5] = di * u1; Az[s5] = di * u2;
+                    }
+                    float cum = 0.f, dsum = 0.f;
+                    const bool inside = I >= 2 && I + 2 < M && J >= 2 && J + 2 < M;
+#pragma unroll 1
+                    for (int jj = 0; jj < 5; jj++) {
+                        const float dj = (float)(J - 2 + jj - ci);
+                        const float Bx = dj * v0, By = dj * v1, Bz = dj * v2;
+                        float val[5];
+                        const int row = (J - 2 + jj) * M + (I - 2);
+                        if (inside) {
+                            f4 a;
+                            __builtin_memcpy(&a, im + row, 16);
+                            val[0] = a.x; val[1] = a.y; val[2] = a.z; val[3] = a.w;
+                            val[4] = im[row + 4];
+                        } else {
+                            // Reference bound on the linear index only
+                            // (gpuBackProject.cu:99): wraps across rows.
+#pragma unroll
+                            for (int ii = 0; ii < 5; ii++) {
+                                const int lin = row + ii;
+                                val[ii] = (lin >= 0 && lin < M2) ? im[lin] : 0.f;
+                            }
+                        }
 #pragma unroll
                         for (int ii = 0; ii < 5; ii++) {
-                            const int lin = row + ii;
-                            val[ii] = (lin >= 0 && lin < M2) ? im[lin] : 0.f;
+                            const float px = Ax[ii] + Bx + cvf;
+                            const float py = Ay[ii] + By + cvf;
+                            const float pz = Az[ii] + Bz + cvf;
+                            float w = kb_fast(sker, px - fvi);
+                            w = w * kb_fast(sker, py - fvj);
+                            w = w * kb_fast(sker, pz - fvk);
+                            cum = fmaf(val[ii], w, cum);
+                            if (DENS) {
+                                const int lin = row + ii;
+                                dsum += (inside || (lin >= 0 && lin < M2)) ? w : 0.f;
+                            }
                         }
                     }
-#pragma unroll
-                    for (int ii = 0; ii < 5; ii++) {
-                        const float px = Ax[ii] + Bx[jj] + cvf;
-                        const float py = Ay[ii] + By[jj] + cvf;
-                        const float pz = Az[ii] + Bz[jj] + cvf;
-                        float w = kb_fast(sker, px - fvi);
-                        w = w * kb_fast(sker, py - fvj);
-                        w = w * kb_fast(sker, pz - fvk);
-                        cum = fmaf(val[ii], w, cum);
-                        if (DENS) {
-                            const int lin = row + ii;
-                            dsum += (inside || (lin >= 0 && lin < M2)) ? w : 0.f;
-                        }
-                    }
+                    // Two queued entries of one group can name the same voxel
+                    // (different directions): LDS atomic add.
+                    atomicAdd(&sacc[wv][vox], cum);
+                    if (DENS) atomicAdd(&sden[wv][vox], dsum);
                 }
-                sacc[wv][vox] += cum;
-                if (DENS) sden[wv][vox] += dsum;
             }
+            wave_lds_sync();  // queue slots consumed before they are refilled
+            if (last) break;
         }
     }
     __syncthreads();

@@ -120,6 +120,21 @@ def test_back_project_reference(cuda, oracle, ker32, n, p, seed):
     assert_parity(_np(dens), ref_d)
 
 
+def test_back_project_repeatable(cuda, oracle, ker32):
+    """Two identical back projections give bit-identical volumes."""
+    import torch
+
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.synth import random_directions
+
+    m, v, p = 64, 70, 300
+    axes = _t(random_directions(p, 8), cuda)
+    y = _t(np.random.default_rng(8).standard_normal((p, m, m)).astype(np.float32), cuda)
+    a = ops.back_project(y, axes, v, ker=_t(ker32, cuda))
+    b = ops.back_project(y, axes, v, ker=_t(ker32, cuda))
+    assert torch.equal(a, b)
+
+
 @pytest.mark.parametrize("ksize", [301, 1001])
 def test_back_project_generic_table(cuda, oracle, ksize):
     """Tables other than 501 entries take the generic (voxel-per-lane) kernel."""
