@@ -381,6 +381,37 @@ void check_launch(hipError_t e, const char *what)
     if (e != hipSuccess) raise(MGG_ERR_HIP, "%s launch failed: %s", what, hipGetErrorString(e));
 }
 
+// Centre-out brick order of the fast back projection, one device copy per
+// (device, V), built on first use and kept for the process.
+std::mutex g_order_mu;
+std::map<std::pair<int, int>, std::unique_ptr<DevBuf>> g_order;
+
+const int *brick_order(const ProjGeom &g, int &len)
+{
+    len = 0;
+    if (!bp_uses_fast_path(g)) return nullptr;
+    len = bp_brick_count(g.V, true);
+    std::lock_guard<std::mutex> lk(g_order_mu);
+    auto &b = g_order[std::make_pair(current_device(), g.V)];
+    if (!b) {
+        std::vector<int> host(len);
+        bp_brick_order(g.V, host.data());
+        auto nb = std::make_unique<DevBuf>();
+        nb->reserve(sizeof(int) * len);
+        HIP_TRY(hipMemcpy(nb->ptr, host.data(), sizeof(int) * len, hipMemcpyHostToDevice));
+        b = std::move(nb);
+    }
+    return b->as<int>();
+}
+
+hipError_t back_project_launch(float *vol, float *dens, const ProjGeom &g, const float *axes, int n,
+                               const float *casImgs, hipStream_t s)
+{
+    int len = 0;
+    const int *order = brick_order(g, len);
+    return launch_back_project(vol, dens, g, axes, n, casImgs, s, order, len);
+}
+
 // CAS images (n x m^2) -> images (n x nn^2); casImgs is reused as the C2R output.
 void chain_cas_to_images(float *casImgs, float2 *herm, float *images, int n, int m, int nn,
                          hipStream_t s)
@@ -716,7 +747,7 @@ struct mgg_gridder {
                 HIP_TRY(hipMemcpyAsync(s.casImgs.ptr, casIn + (size_t)c.first * M * M,
                                        sizeof(float) * (size_t)c.count * M * M, hipMemcpyHostToDevice, s.stream));
             }
-            check_launch(launch_back_project(parts + V3 * c.slot, withDensity ? partsDens + V3 * c.slot : nullptr, g,
+            check_launch(back_project_launch(parts + V3 * c.slot, withDensity ? partsDens + V3 * c.slot : nullptr, g,
                                              s.axes.as<float>(), c.count, s.casImgs.as<float>(), s.stream),
                          "back_project");
         }
@@ -1137,7 +1168,7 @@ int mgg_op_back_project(float *cas_volume, float *density, int v, const float *a
         validate_geom(v, m, ker_size, ker_hwidth);
         if (n_axes < 0) raise(MGG_ERR_INVALID_ARGUMENT, "n_axes < 0");
         ProjGeom g{v, m, mask_radius, ker, ker_size, ker_hwidth};
-        check_launch(launch_back_project(cas_volume, density, g, axes, n_axes, cas_images, as_stream(stream)),
+        check_launch(back_project_launch(cas_volume, density, g, axes, n_axes, cas_images, as_stream(stream)),
                      "back_project");
     });
 }

@@ -36,7 +36,13 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
 // projection of all-ones CAS images) are accumulated into.
 hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
                                const float *axes, int nAxes, const float *casImgs,
-                               hipStream_t s);
+                               hipStream_t s, const int *brickOrder, int orderLen);
+// The fast back projection walks bricks in a centre-out order; these give the
+// table length (0 when the generic kernel is used) and fill it (host memory).
+int bp_brick_count(int V, bool fast);
+void bp_brick_order(int V, int *out);
+// True when launch_back_project takes the fast path for this geometry.
+bool bp_uses_fast_path(const ProjGeom &g);
 // Exact transpose of launch_forward_project (atomic scatter into volAcc).
 hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const float *axes,
                                        int nAxes, const float *casImgs, hipStream_t s);

@@ -15,6 +15,8 @@
 
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 namespace mgg {
 namespace {
 
@@ -284,6 +286,7 @@ struct BPArgs {
     float maskR, kerScale, kerCenter, fW, hw;
     int convW, bricksX, bricksXY;
     int maskD2;  // largest integer d2 with sqrtf((float)d2) <= maskR (-1: none)
+    const int *order;  // bp_fast_kernel: brick id per work item, heaviest first
     unsigned perXcd, totalWork;
 };
 
@@ -497,8 +500,11 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
     __shared__ float sden[DENS ? 4 : 1][DENS ? BV : 1];
     __shared__ unsigned short squeue[4][QCAP];
 
-    const unsigned work = xcd_work(p.perXcd);
-    if (work >= p.totalWork) return;
+    // Bricks near the centre meet every direction and bricks near the mask edge
+    // few, so the order table puts the heavy (central) bricks first and
+    // consecutive work items go round-robin to the XCDs.
+    if (blockIdx.x >= p.totalWork) return;
+    const unsigned work = (unsigned)p.order[blockIdx.x];
     const int bz = (int)(work / (unsigned)p.bricksXY);
     const int rem = (int)(work - (unsigned)bz * (unsigned)p.bricksXY);
     const int by = rem / p.bricksX, bx = rem - by * p.bricksX;
@@ -1156,6 +1162,43 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
     return hipGetLastError();
 }
 
+bool bp_uses_fast_path(const ProjGeom &g)
+{
+    float kerScale, kerCenter, fW;
+    int convW;
+    proj_consts(g, kerScale, kerCenter, fW, convW);
+    return convW == 2 && kerScale == 125.0f && kerCenter == 250.0f && g.K == KFAST && g.V <= 8192;
+}
+
+int bp_brick_count(int V, bool fast)
+{
+    const int b = fast ? (V + BB - 1) / BB : 0;
+    return b * b * b;
+}
+
+void bp_brick_order(int V, int *out)
+{
+    // Bricks of edge BB sorted by the distance of their centre from the volume
+    // centre (ties by index): central bricks, which every direction's slab
+    // crosses, are dispatched first.
+    const int nb = (V + BB - 1) / BB;
+    const double c = V / 2;
+    struct Item { double d; int id; };
+    Item *items = new Item[(size_t)nb * nb * nb];
+    int n = 0;
+    for (int z = 0; z < nb; z++)
+        for (int y = 0; y < nb; y++)
+            for (int x = 0; x < nb; x++) {
+                const double dx = x * BB + 0.5 * (BB - 1) - c, dy = y * BB + 0.5 * (BB - 1) - c,
+                             dz = z * BB + 0.5 * (BB - 1) - c;
+                items[n] = {dx * dx + dy * dy + dz * dz, (z * nb + y) * nb + x};
+                n++;
+            }
+    std::sort(items, items + n, [](const Item &a, const Item &b) { return a.d < b.d || (a.d == b.d && a.id < b.id); });
+    for (int i = 0; i < n; i++) out[i] = items[i].id;
+    delete[] items;
+}
+
 hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const float *axes,
                                        int nAxes, const float *casImgs, hipStream_t s)
 {
@@ -1176,7 +1219,8 @@ hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const f
 }
 
 hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
-                               const float *axes, int nAxes, const float *casImgs, hipStream_t s)
+                               const float *axes, int nAxes, const float *casImgs, hipStream_t s,
+                               const int *brickOrder, int orderLen)
 {
     if (nAxes <= 0) return hipSuccess;
     BPArgs a;
@@ -1184,8 +1228,7 @@ hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
     a.maskR = g.maskRadius; a.hw = g.kerHWidth;
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
-    const bool fast = a.convW == 2 && a.kerScale == 125.0f && a.kerCenter == 250.0f && g.K == KFAST &&
-                      g.V <= 8192;
+    const bool fast = bp_uses_fast_path(g);
     a.maskD2 = mask_d2(g.maskRadius, g.V);
     const int ex = fast ? BB : BPX, ey = fast ? BB : BPY, ez = fast ? BB : BPZ;
     a.bricksX = (g.V + ex - 1) / ex;
@@ -1194,10 +1237,13 @@ hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
     a.bricksXY = a.bricksX * bricksY;
     a.totalWork = (unsigned)a.bricksXY * (unsigned)bricksZ;
     a.perXcd = (a.totalWork + 7) / 8;
+    a.order = brickOrder;
     const dim3 grid(a.perXcd * 8), block(kThreads);
     if (fast) {
-        if (densAcc) hipLaunchKernelGGL(bp_fast_kernel<true>, grid, block, 0, s, a);
-        else hipLaunchKernelGGL(bp_fast_kernel<false>, grid, block, 0, s, a);
+        if (!brickOrder || orderLen != (int)a.totalWork) return hipErrorInvalidValue;
+        const dim3 g1(a.totalWork);
+        if (densAcc) hipLaunchKernelGGL(bp_fast_kernel<true>, g1, block, 0, s, a);
+        else hipLaunchKernelGGL(bp_fast_kernel<false>, g1, block, 0, s, a);
     } else {
         if (densAcc) hipLaunchKernelGGL(bp_kernel<true>, grid, block, 0, s, a);
         else hipLaunchKernelGGL(bp_kernel<false>, grid, block, 0, s, a);
