@@ -222,6 +222,233 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
 }
 
 // ---------------------------------------------------------------------------
+// Forward projection, LDS-staged (kerHWidth 2).  A workgroup is 4 waves; each
+// wave owns one 8x8 pixel tile of one direction.  The wave stages into LDS the
+// axis-aligned box of volume voxels its pixels' 5x5x5 stencils can touch
+// (bounded from the tile's four corner positions, about 30 voxels per pixel),
+// reading each box row as unaligned 16-byte segments, and gathers the 125 taps
+// from LDS instead of issuing 25 scattered row loads per pixel to the texture
+// path.  A tile whose box exceeds the per-wave LDS budget (unusual geometry
+// only) gathers from global memory.  The gather is arranged as
+// acc += wk[c] * sum_ab (wi[a]*wj[b]) * vol, i.e. the reference's product
+// (wi*wj)*wk regrouped: rounding decisions are unchanged, only FP32
+// accumulation rounding differs.
+// ---------------------------------------------------------------------------
+constexpr int FPT = 8;          // tile edge (pixels)
+constexpr int FPBOX = 3072;     // per-wave staged box budget (floats)
+constexpr int FPKER = 1152;     // table floats kept in LDS (padded 501-entry table fits)
+
+__device__ __forceinline__ void wave_lds_sync_fp()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// Padded 501-entry table lookup (see bp_fast_kernel for the exactness argument).
+constexpr int FKPAD = 320;
+__device__ __forceinline__ float kb_fast_fp(const float *__restrict__ sk, float r)
+{
+    float s = r * 125.0f;
+    s = s + 250.0f;
+    int i;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(s));
+    i = min(max(i, -FKPAD), 500 + FKPAD);
+    return sk[FKPAD + i];
+}
+
+template <typename L>
+__device__ __forceinline__ void load4(const typename L::T *p, float (&v)[4]);
+template <>
+__device__ __forceinline__ void load4<VolLoad<DT_F32>>(const float *p, float (&v)[4])
+{
+    f4 a;
+    __builtin_memcpy(&a, p, 16);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+template <>
+__device__ __forceinline__ void load4<VolLoad<DT_BF16>>(const uint16_t *p, float (&v)[4])
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    u2 a;
+    __builtin_memcpy(&a, p, 8);
+    v[0] = __uint_as_float(a.x << 16); v[1] = __uint_as_float(a.x & 0xffff0000u);
+    v[2] = __uint_as_float(a.y << 16); v[3] = __uint_as_float(a.y & 0xffff0000u);
+}
+template <>
+__device__ __forceinline__ void load4<VolLoad<DT_F16>>(const uint16_t *p, float (&v)[4])
+{
+    typedef unsigned int u2 __attribute__((ext_vector_type(2)));
+    u2 a;
+    __builtin_memcpy(&a, p, 8);
+    v[0] = VolLoad<DT_F16>::h2f(a.x); v[1] = VolLoad<DT_F16>::h2f(a.x >> 16);
+    v[2] = VolLoad<DT_F16>::h2f(a.y); v[3] = VolLoad<DT_F16>::h2f(a.y >> 16);
+}
+
+template <int DT, bool FAST>
+__device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__restrict__ sker, float *__restrict__ box,
+                                            unsigned work, int ln)
+{
+    using L = VolLoad<DT>;
+    using T = typename L::T;
+    const int a = (int)(work / (unsigned)p.tilesPerImg);
+    const int tile = (int)(work - (unsigned)a * (unsigned)p.tilesPerImg);
+    const int ty = tile / p.tilesPerRow, tx = tile - ty * p.tilesPerRow;
+    const int i0 = tx * FPT, j0 = ty * FPT;
+    const int i = i0 + (ln & 7), j = j0 + (ln >> 3);
+    const int M = p.M, V = p.V, ci = M / 2, cv = V / 2;
+    const bool pix = i < M && j < M;
+
+    const float r = sqrtf((float)(i - ci) * (i - ci) + (j - ci) * (j - ci));
+    const bool active = pix && r <= p.maskR;
+    float *out = p.out + (size_t)a * M * M + (size_t)j * M + i;
+    if (__ballot(active) == 0ull) {
+        if (pix) *out = 0.f;
+        return;
+    }
+    const float *ax = p.axes + (size_t)9 * a;
+    const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
+    const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
+    const float fi = (float)(i - ci), fj = (float)(j - ci);
+    const float x = u0 * fi + v0 * fj + (float)cv;
+    const float y = u1 * fi + v1 * fj + (float)cv;
+    const float z = u2 * fi + v2 * fj + (float)cv;
+    const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
+
+    // Box from the tile corners: positions are affine in (i, j), so every
+    // pixel's rounded coordinate lies in [floor(min corner), ceil(max corner)]
+    // (the 1e-3 margin covers FP32 evaluation error), widened by the stencil.
+    const float ai = (float)(i0 - ci), bi = (float)(i0 + FPT - 1 - ci);
+    const float aj = (float)(j0 - ci), bj = (float)(j0 + FPT - 1 - ci);
+    auto lo = [&](float uu, float vv) {
+        return (int)floorf(fminf(uu * ai, uu * bi) + fminf(vv * aj, vv * bj) + (float)cv - 1e-3f) - 2;
+    };
+    auto hi = [&](float uu, float vv) {
+        return (int)ceilf(fmaxf(uu * ai, uu * bi) + fmaxf(vv * aj, vv * bj) + (float)cv + 1e-3f) + 2;
+    };
+    const int bx0 = __builtin_amdgcn_readfirstlane(lo(u0, v0));
+    const int by0 = __builtin_amdgcn_readfirstlane(lo(u1, v1));
+    const int bz0 = __builtin_amdgcn_readfirstlane(lo(u2, v2));
+    const int Dx = __builtin_amdgcn_readfirstlane(hi(u0, v0)) - bx0 + 1;
+    const int Dy = __builtin_amdgcn_readfirstlane(hi(u1, v1)) - by0 + 1;
+    const int Dz = __builtin_amdgcn_readfirstlane(hi(u2, v2)) - bz0 + 1;
+    const int segs = (Dx + 3) >> 2;  // 16-byte segments per box row
+    const int RP = segs * 4;         // LDS row pitch (floats), 16-byte aligned
+    const long long V3 = (long long)V * V * V;
+    const T *vol = (const T *)p.vol;
+
+    float wi[5], wj[5], wk[5];
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+        if (FAST) {
+            wi[t] = kb_fast_fp(sker, (float)(X - 2 + t) - x);
+            wj[t] = kb_fast_fp(sker, (float)(Y - 2 + t) - y);
+            wk[t] = kb_fast_fp(sker, (float)(Z - 2 + t) - z);
+        } else {
+            wi[t] = sker[kb_index((float)(X - 2 + t) - x, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wj[t] = sker[kb_index((float)(Y - 2 + t) - y, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wk[t] = sker[kb_index((float)(Z - 2 + t) - z, p.fW, p.kerScale, p.kerCenter, p.K)];
+        }
+    }
+    float wij[5][5];
+#pragma unroll
+    for (int aa = 0; aa < 5; aa++)
+#pragma unroll
+        for (int bb = 0; bb < 5; bb++) wij[aa][bb] = wi[aa] * wj[bb];
+
+    float acc = 0.f;
+    if (Dx >= 5 && Dy >= 5 && Dz >= 5 && Dx <= 64 && Dy <= 64 && Dz <= 64 && RP * Dy * Dz <= FPBOX &&
+        V3 < (1LL << 31)) {
+        // Stage: lane -> (row slot, segment); 64 / segs rows per pass.
+        const int rowsPer = 64 / segs;
+        const int rIn = ln / segs, sg = ln - rIn * segs;
+        const int nRows = Dy * Dz;
+        const int VV = V * V, V3i = (int)V3;
+        const float invDy = 1.0f / (float)Dy;
+        // Four passes per batch: all loads of a batch are in flight before
+        // the first LDS store waits on them.
+        constexpr int SB = 4;
+        for (int r0 = 0; r0 < nRows; r0 += SB * rowsPer) {
+            float v4[SB][4];
+            int dst[SB];
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                const int row = r0 + q * rowsPer + rIn;
+                dst[q] = -1;
+                if (rIn < rowsPer && row < nRows) {
+                    // row / Dy: (row + 0.5) / Dy is at least 1/(2 Dy) from an integer
+                    const int ez = (int)(((float)row + 0.5f) * invDy), ey = row - ez * Dy;
+                    const int lin = (bz0 + ez) * VV + (by0 + ey) * V + (bx0 + 4 * sg);
+                    if (lin >= 0 && lin + 3 < V3i) {
+                        load4<L>(vol + lin, v4[q]);
+                    } else {
+                        // the reference's bound on the linear index (gpuForwardProject.cu:90)
+#pragma unroll
+                        for (int e = 0; e < 4; e++)
+                            v4[q][e] = (lin + e >= 0 && lin + e < V3i) ? L::one(vol + lin + e) : 0.f;
+                    }
+                    dst[q] = row * RP + 4 * sg;
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < SB; q++) {
+                if (dst[q] >= 0) {
+                    f4 w4 = {v4[q][0], v4[q][1], v4[q][2], v4[q][3]};
+                    *(f4 *)(box + dst[q]) = w4;
+                }
+            }
+        }
+        wave_lds_sync_fp();
+        if (active) {
+            const float *bp = box + ((Z - 2 - bz0) * Dy + (Y - 2 - by0)) * RP + (X - 2 - bx0);
+#pragma unroll
+            for (int cc = 0; cc < 5; cc++) {
+                float sc = 0.f;
+#pragma unroll
+                for (int bb = 0; bb < 5; bb++) {
+                    const float *rp = bp + (cc * Dy + bb) * RP;
+#pragma unroll
+                    for (int aa = 0; aa < 5; aa++) sc = fmaf(wij[aa][bb], rp[aa], sc);
+                }
+                acc = fmaf(wk[cc], sc, acc);
+            }
+        }
+    } else if (active) {
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++) {
+            float sc = 0.f;
+#pragma unroll
+            for (int bb = 0; bb < 5; bb++)
+#pragma unroll
+                for (int aa = 0; aa < 5; aa++) {
+                    const long long lin = (long long)(Z - 2 + cc) * V * V + (long long)(Y - 2 + bb) * V + (X - 2 + aa);
+                    if (lin < V3 && lin >= 0) sc = fmaf(wij[aa][bb], L::one(vol + lin), sc);
+                }
+            acc = fmaf(wk[cc], sc, acc);
+        }
+    }
+    if (pix) *out = active ? acc : 0.f;
+}
+
+template <int DT, bool FAST>
+__global__ __launch_bounds__(kThreads) void fp_lds_kernel(FPArgs p)
+{
+    __shared__ float sker[FPKER];
+    __shared__ __attribute__((aligned(16))) float sbox[4][FPBOX];
+
+    if (FAST) {
+        for (int t = threadIdx.x; t < 501 + 2 * FKPAD; t += kThreads)
+            sker[t] = p.ker[min(max(t - FKPAD, 0), 500)];
+    } else {
+        for (int t = threadIdx.x; t < p.K; t += kThreads) sker[t] = p.ker[t];
+    }
+    __syncthreads();
+    // Persistent: each wave walks work items wv + 4*blockIdx.x, + 4*gridDim.x, ...
+    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
+        fp_lds_tile<DT, FAST>(p, sker, sbox[wv], work, ln);
+}
+
+// ---------------------------------------------------------------------------
 // Exact transpose of the forward gather: each in-mask pixel scatters its
 // value through the same (voxel, weight) pairs with FP32 atomics.
 // ---------------------------------------------------------------------------
@@ -1135,6 +1362,37 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
     a.maskR = g.maskRadius;
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
+    const bool w2 = a.convW == 2 && g.K <= FPKER;
+    if (w2) {
+        // LDS-staged kernel: 8x8 tiles, one per wave, 4 per workgroup.
+        const bool fast = a.kerScale == 125.0f && a.kerCenter == 250.0f && g.K == 501;
+        a.tilesPerRow = (g.M + FPT - 1) / FPT;
+        a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
+        const unsigned long long work = (unsigned long long)a.tilesPerImg * nAxes;
+        if (work > 0xffffffffull / 2) return hipErrorInvalidValue;
+        a.totalWork = (unsigned)work;
+        a.perXcd = 0;
+        // persistent grid: a few resident workgroups per CU
+        static int cus = 0;
+        if (!cus) {
+            int dev = 0;
+            if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
+                cus = 256;
+        }
+        const unsigned groups = (a.totalWork + 3) / 4;
+        const dim3 grid(std::min<unsigned>(groups, (unsigned)cus * 3u)), block(kThreads);
+#define MGG_FP_LAUNCH(DTV)                                                                   \
+    if (fast) hipLaunchKernelGGL((fp_lds_kernel<DTV, true>), grid, block, 0, s, a);          \
+    else hipLaunchKernelGGL((fp_lds_kernel<DTV, false>), grid, block, 0, s, a);
+        switch (dtype) {
+        case DT_F32: MGG_FP_LAUNCH(DT_F32) break;
+        case DT_BF16: MGG_FP_LAUNCH(DT_BF16) break;
+        case DT_F16: MGG_FP_LAUNCH(DT_F16) break;
+        default: return hipErrorInvalidValue;
+        }
+#undef MGG_FP_LAUNCH
+        return hipGetLastError();
+    }
     a.tilesPerRow = (g.M + 15) / 16;
     a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
     const unsigned long long work = (unsigned long long)a.tilesPerImg * nAxes;
@@ -1142,22 +1400,21 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
     a.totalWork = (unsigned)work;
     a.perXcd = (a.totalWork + 7) / 8;
     const dim3 grid(a.perXcd * 8), block(kThreads);
-    const bool w2 = a.convW == 2;
+    const bool rows = a.convW == 2;  // 5-tap row loads (tables too large for the staged kernel)
     switch (dtype) {
     case DT_F32:
-        if (w2) hipLaunchKernelGGL((fp_kernel<DT_F32, true>), grid, block, 0, s, a);
+        if (rows) hipLaunchKernelGGL((fp_kernel<DT_F32, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((fp_kernel<DT_F32, false>), grid, block, 0, s, a);
         break;
     case DT_BF16:
-        if (w2) hipLaunchKernelGGL((fp_kernel<DT_BF16, true>), grid, block, 0, s, a);
+        if (rows) hipLaunchKernelGGL((fp_kernel<DT_BF16, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((fp_kernel<DT_BF16, false>), grid, block, 0, s, a);
         break;
     case DT_F16:
-        if (w2) hipLaunchKernelGGL((fp_kernel<DT_F16, true>), grid, block, 0, s, a);
+        if (rows) hipLaunchKernelGGL((fp_kernel<DT_F16, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((fp_kernel<DT_F16, false>), grid, block, 0, s, a);
         break;
-    default:
-        return hipErrorInvalidValue;
+    default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -154,6 +154,21 @@ def test_back_project_generic_table(cuda, oracle, ksize):
     assert_parity(got, oracle.forward_project(_np(out), axes, ker, m))
 
 
+@pytest.mark.parametrize("dtype", ["float32", "bfloat16"])
+def test_forward_project_large_table(cuda, oracle, dtype):
+    """A 1501-entry table does not fit the staged kernel's LDS table: the
+    direct-gather kernel with 5-tap row loads runs instead."""
+    import torch
+
+    from multigpugridder_amd import ops
+
+    ker = oracle.kb_table(2, 1501).astype(np.float32)
+    vol, cas, axes, m, v = _setup(16, 7, 9, oracle)
+    dv = _t(cas, cuda).to(getattr(torch, dtype))
+    got = _np(ops.forward_project(dv, _t(axes, cuda), m, ker=_t(ker, cuda)))
+    assert_parity(got, oracle.forward_project(_np(dv.float()), axes, ker, m))
+
+
 def test_back_project_accumulates_and_wraps(cuda, oracle, ker32):
     """Two launches accumulate; the linear-index wrap quirk is reproduced."""
     import torch
