@@ -23,7 +23,9 @@
  *   2. device-pointer operators (mgg_op_*): the same kernels on caller-owned
  *      device memory and a caller stream, no host staging;
  *   3. the legacy symbols the reference's ctypes wrapper binds
- *      (Python/MultiGPUGridder_Python.py:11-48).
+ *      (Python/MultiGPUGridder_Python.py:11-48);
+ *   4. a request server (mgg_server_*): resident volumes and continuous
+ *      batching of forward / back projection requests.
  */
 #ifndef MGG_H
 #define MGG_H
@@ -43,6 +45,8 @@ extern "C" {
 #define MGG_ERR_HIP 5              /* a HIP runtime call failed */
 #define MGG_ERR_FFT 6              /* rocFFT plan or execution failed */
 #define MGG_ERR_UNSUPPORTED 7      /* valid request this build does not do */
+#define MGG_ERR_TIMEOUT 8          /* mgg_server_wait: request not finished in time */
+#define MGG_ERR_BUSY 9             /* mgg_server_submit: request queue full */
 
 /* Storage types a device CAS volume can be gathered from (mgg_op_forward_project). */
 #define MGG_DTYPE_F32 0
@@ -245,6 +249,95 @@ void *SetImgSize(void *h, int *dims);
 void *SetMaskRadius(void *h, float *radius);
 void *Forward_Project(void *h);
 void *Back_Project(void *h);
+
+/* ======================================================================== */
+/* 4. Request server                                                        */
+/* ======================================================================== */
+/* No counterpart in the reference, whose callers run one synchronous
+ * forwardProject / backProject per request (MultiGPUGridder_Matlab_Class.m:198,
+ * :245).  The server keeps CAS volumes resident in HBM and runs requests of
+ * mixed N and direction counts:
+ *   - admission: each batch's device memory comes from one arena of
+ *     budget_bytes; a batch starts when a pipeline is free and it fits;
+ *   - batching: forward requests on one volume share launches of up to
+ *     max_batch_directions directions; a back request owns its accumulator;
+ *   - overlap: num_pipelines batches in flight, one stream each.
+ * Results equal the plain path's for the same request: a forward request
+ * returns forwardProject's images of its directions (n_axes x N x N); a back
+ * request returns getVol of the back projection of its images into a zeroed
+ * volume (N^3).  A result does not depend on the other requests in flight.
+ * Array arguments may be pageable host memory, pinned host memory
+ * (mgg_host_alloc; copies then run asynchronously) or device memory; they
+ * must stay valid until mgg_server_wait returns for the request.
+ * Thread safe: any thread may submit and wait. */
+typedef struct mgg_server mgg_server;
+
+#define MGG_REQ_FORWARD 0
+#define MGG_REQ_BACK 1
+
+typedef struct mgg_server_config {
+    int device;               /* HIP device ordinal */
+    size_t budget_bytes;      /* device memory for volumes + in-flight batches
+                                 (0: half the free memory, at most 32 GiB) */
+    int max_batch_directions; /* directions per launch (0: 8192) */
+    int num_pipelines;        /* batches in flight (0: 4) */
+    int max_queued;           /* waiting requests before submit says BUSY (0: no limit) */
+} mgg_server_config;
+
+typedef struct mgg_request {
+    int kind;          /* MGG_REQ_FORWARD or MGG_REQ_BACK */
+    int volume_id;     /* from mgg_server_add_volume */
+    int n_axes;        /* directions */
+    const float *axes; /* 9 * n_axes */
+    float *images;     /* forward: output n_axes x N^2; back: input */
+    float *volume;     /* back: output N^3; forward: unused (NULL) */
+} mgg_request;
+
+typedef struct mgg_request_info {
+    int status;      /* MGG_OK or the request's failure code */
+    double t_submit; /* seconds on the server clock (mgg_server_clock) */
+    double t_start;  /* first batch of the request started */
+    double t_done;   /* result written to the caller's arrays */
+} mgg_request_info;
+
+typedef struct mgg_server_stats {
+    unsigned long long submitted, completed, failed;
+    unsigned long long batches;            /* launches of the projection kernels */
+    unsigned long long batched_directions; /* directions over all batches */
+    unsigned long long merged_batches;     /* forward batches holding several requests */
+    size_t budget_bytes, resident_bytes, in_use_bytes, peak_bytes;
+    int queued, in_flight, volumes;
+} mgg_server_stats;
+
+/* cfg may be NULL (all defaults, device 0). */
+int mgg_server_create(const mgg_server_config *cfg, mgg_server **out);
+/* Finishes every submitted request, then frees the server. */
+int mgg_server_destroy(mgg_server *s);
+/* volume: N^3 host floats (x fastest).  The CAS volume (CASFromVol_Gridder.m)
+ * is built on the device and stays resident; MGG_ERR_OUT_OF_MEMORY if the
+ * budget cannot hold it. */
+int mgg_server_add_volume(mgg_server *s, const float *volume, int volume_size, float interp_factor,
+                          int extra_padding, int *volume_id);
+/* Waits until no request uses the volume, then frees it. */
+int mgg_server_remove_volume(mgg_server *s, int volume_id);
+/* Queues a request; *ticket identifies it for mgg_server_wait.  Fails at once
+ * (nothing queued) on bad arguments, an unknown volume, a request larger than
+ * the budget (MGG_ERR_OUT_OF_MEMORY) or a full queue (MGG_ERR_BUSY). */
+int mgg_server_submit(mgg_server *s, const mgg_request *req, unsigned long long *ticket);
+/* Waits for a request (timeout_s < 0: no limit) and returns its status; the
+ * ticket is consumed unless the wait times out (MGG_ERR_TIMEOUT). */
+int mgg_server_wait(mgg_server *s, unsigned long long ticket, double timeout_s, mgg_request_info *info);
+int mgg_server_get_stats(mgg_server *s, mgg_server_stats *stats);
+/* hold != 0: submitted requests queue up and no new batch starts (running
+ * batches finish); hold = 0 lets the scheduler batch everything queued.  A
+ * caller uses it to hand over a group of requests that should share launches.
+ * mgg_server_destroy releases a hold. */
+int mgg_server_hold(mgg_server *s, int hold);
+/* Seconds since the server was created (the clock of mgg_request_info). */
+double mgg_server_clock(mgg_server *s);
+/* Pinned host memory, so request transfers run asynchronously. */
+int mgg_host_alloc(size_t bytes, void **ptr);
+int mgg_host_free(void *ptr);
 
 #ifdef __cplusplus
 }

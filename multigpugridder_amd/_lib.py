@@ -31,12 +31,14 @@ MGG_ERR_OUT_OF_MEMORY = 4
 MGG_ERR_HIP = 5
 MGG_ERR_FFT = 6
 MGG_ERR_UNSUPPORTED = 7
+MGG_ERR_TIMEOUT = 8
+MGG_ERR_BUSY = 9
 
 DTYPE_F32, DTYPE_BF16, DTYPE_F16 = 0, 1, 2
 
 _ERR_NAMES = {
     1: "invalid argument", 2: "not set", 3: "device", 4: "out of memory",
-    5: "HIP", 6: "FFT", 7: "unsupported",
+    5: "HIP", 6: "FFT", 7: "unsupported", 8: "timeout", 9: "busy",
 }
 
 
@@ -114,6 +116,18 @@ _SIGS = {
     "mgg_kb_table": [_vp, _ci, _ci],
     "mgg_precomp": [_vp, _ci, _ci],
     "mgg_uniform_axes": [_vp, _ci, _ci],
+    # request server
+    "mgg_server_create": [_vp, ctypes.POINTER(_vp)],
+    "mgg_server_destroy": [_vp],
+    "mgg_server_add_volume": [_vp, _vp, _ci, _cf, _ci, _ip],
+    "mgg_server_remove_volume": [_vp, _ci],
+    "mgg_server_submit": [_vp, _vp, ctypes.POINTER(ctypes.c_ulonglong)],
+    "mgg_server_wait": [_vp, ctypes.c_ulonglong, _cd, _vp],
+    "mgg_server_get_stats": [_vp, _vp],
+    "mgg_server_hold": [_vp, _ci],
+    "mgg_server_clock": [_vp],
+    "mgg_host_alloc": [_sz, ctypes.POINTER(_vp)],
+    "mgg_host_free": [_vp],
     # legacy ctypes symbols (Python/MultiGPUGridder_Python.py:11-48)
     "Gridder_new": [],
     "SetNumberGPUs": [_vp, _ci],
@@ -133,6 +147,7 @@ _RESTYPES = {
     "mgg_last_error": ctypes.c_char_p,
     "mgg_version": ctypes.c_char_p,
     "mgg_get_mask_radius": _cf,
+    "mgg_server_clock": _cd,
     "Gridder_new": _vp, "SetNumberGPUs": _vp, "SetNumberStreams": _vp, "SetVolume": _vp,
     "GetVolume": _fp, "ResetVolume": _vp, "SetImages": _vp, "GetImages": _fp, "SetAxes": _vp,
     "SetImgSize": _vp, "SetMaskRadius": _vp, "Forward_Project": _vp, "Back_Project": _vp,
