@@ -93,6 +93,186 @@ def cpu_baseline(n: int, p_sample: int, seed: int = 1):
     }
 
 
+# ---------------------------------------------------------------------------
+# Serving: plain per-request calls vs the request server
+# ---------------------------------------------------------------------------
+def make_requests(count, sizes, seed, lo=64, hi=8192):
+    """Request mix: 60% forward / 40% back, direction count log-uniform in
+    [lo, hi], volume drawn uniformly from ``sizes`` (list of volume keys)."""
+    rng = np.random.default_rng(seed)
+    reqs = []
+    for _ in range(count):
+        kind = "fp" if rng.random() < 0.6 else "bp"
+        k = int(round(float(np.exp(rng.uniform(np.log(lo), np.log(hi))))))
+        reqs.append((kind, int(rng.integers(len(sizes))), k))
+    return reqs
+
+
+class PlainPath:
+    """The synchronous drop-in path, one gridder-object call sequence per
+    request: forward = SetCoordAxes / SetImages / ForwardProject on an object
+    whose CAS volume is resident; back = SetImages / BackProject (the object
+    zeroes its volume first) / getVol."""
+
+    def __init__(self, vols, max_axes):
+        import ctypes
+        from multigpugridder_amd import _lib
+        self._lib, self._ct = _lib, ctypes
+        self.objs = []
+        for vol in vols:
+            n = vol.shape[0]
+            pair = []
+            for _ in range(2):
+                h = ctypes.c_void_p()
+                _lib.call("mgg_create", n, max_axes, 2.0, 3, 1, None, 1, 0, ctypes.byref(h))
+                pair.append(h)
+            _lib.call("mgg_set_volume", pair[0], vol.ctypes.data, _lib.int_array([n, n, n]))
+            _lib.call("mgg_volume_to_cas_volume", pair[0])
+            self.objs.append((n, pair[0], pair[1]))
+
+    def run(self, kind, vi, axes, k, images, volume):
+        L = self._lib
+        n, hfp, hbp = self.objs[vi]
+        h = hfp if kind == "fp" else hbp
+        L.call("mgg_set_coord_axes", h, axes.ctypes.data, L.int_array([9 * k, 1]))
+        L.call("mgg_set_num_axes", h, k)
+        L.call("mgg_set_images", h, images.ctypes.data, L.int_array([n, n, k]))
+        if kind == "fp":
+            L.call("mgg_forward_project", h)
+        else:
+            L.call("mgg_set_volume", h, volume.ctypes.data, L.int_array([n, n, n]))
+            L.call("mgg_back_project", h)
+            L.call("mgg_get_volume", h)
+
+    def close(self):
+        for _, a, b in self.objs:
+            self._lib.lib().mgg_destroy(a)
+            self._lib.lib().mgg_destroy(b)
+
+
+def serving_bench(sizes, n_req, lambdas, seed=100, budget_gb=64, quiet=False):
+    """Capacity (closed loop) and Poisson-arrival latency of the plain path and
+    the server on the same request list.  Returns a dict."""
+    import threading
+    from multigpugridder_amd import Server, pinned_empty
+    from multigpugridder_amd.synth import blob_volume, random_directions
+
+    def log(msg):
+        if not quiet:
+            print(f"[serving] {msg}", file=sys.stderr, flush=True)
+
+    vols = [blob_volume(nn, 10 + i) for i, nn in enumerate(sizes)]
+    reqs = make_requests(n_req, sizes, seed)
+    kmax = max(k for _, _, k in reqs)
+    # shared pinned pools: forward outputs / back inputs per volume size, one volume output
+    pools = {}
+    for nn in set(sizes):
+        pools[nn] = (pinned_empty((kmax, nn, nn)), pinned_empty((kmax, nn, nn)), pinned_empty((nn, nn, nn)))
+        pools[nn][1][:] = np.random.default_rng(nn).standard_normal((kmax, nn, nn), dtype=np.float32)
+    dirs = random_directions(kmax, seed + 1)
+    axes_pool = pinned_empty(dirs.shape)
+    axes_pool[:] = dirs
+    total_dirs = sum(k for _, _, k in reqs)
+    fwd_dirs = sum(k for kind, _, k in reqs if kind == "fp")
+
+    def bufs(kind, vi, k):
+        nn = sizes[vi]
+        fo, bi, vo = pools[nn]
+        return axes_pool[:k], (fo[:k] if kind == "fp" else bi[:k]), vo
+
+    res = {"requests": n_req, "directions": total_dirs, "forward_share": round(fwd_dirs / total_dirs, 3),
+           "volume_sizes": sizes}
+
+    # ---- plain, closed loop (capacity) ----
+    warm = [(kind, vi, kk) for vi in range(len(sizes)) for kind in ("fp", "bp") for kk in (64, 1000)]
+    plain = PlainPath(vols, 1000)
+    for kind, vi, k in warm:  # plans, buffers
+        a, im, vo = bufs(kind, vi, k)
+        plain.run(kind, vi, a, k, im, vo)
+    log("plain closed loop")
+    t0 = time.perf_counter()
+    for kind, vi, k in reqs:
+        a, im, vo = bufs(kind, vi, k)
+        plain.run(kind, vi, a, k, im, vo)
+    t_plain = time.perf_counter() - t0
+    res["plain_dirs_per_s"] = round(total_dirs / t_plain, 1)
+    res["plain_requests_per_s"] = round(n_req / t_plain, 2)
+
+    # ---- server, closed loop ----
+    srv = Server(budget_bytes=int(budget_gb * (1 << 30)))
+    vids = [srv.add_volume(v) for v in vols]
+    for kind, vi, k in warm:
+        a, im, vo = bufs(kind, vi, k)
+        t = srv.submit_forward(vids[vi], a, im) if kind == "fp" else srv.submit_back(vids[vi], a, im, vo)
+        srv.wait(t)
+    log("served closed loop")
+    t0 = srv.clock()
+    tickets = []
+    for kind, vi, k in reqs:
+        a, im, vo = bufs(kind, vi, k)
+        tickets.append(srv.submit_forward(vids[vi], a, im) if kind == "fp" else srv.submit_back(vids[vi], a, im, vo))
+    t_end = max(srv.wait(t).t_done for t in tickets)
+    t_served = t_end - t0
+    res["served_dirs_per_s"] = round(total_dirs / t_served, 1)
+    res["served_requests_per_s"] = round(n_req / t_served, 2)
+    res["served_over_plain"] = round(t_plain / t_served, 3)
+    st = srv.stats()
+    res["served_batches"] = st["batches"]
+    res["served_merged_batches"] = st["merged_batches"]
+
+    # ---- Poisson arrivals at fractions of the plain capacity ----
+    cap = n_req / t_plain
+    points = []
+    for frac in lambdas:
+        lam = frac * cap
+        arr = np.cumsum(np.random.default_rng(seed + 7).exponential(1.0 / lam, n_req))
+        # plain: one synchronous worker, FIFO
+        log(f"plain open loop lambda={lam:.2f}/s")
+        lat_p = []
+        base = time.perf_counter()
+        for i, (kind, vi, k) in enumerate(reqs):
+            now = time.perf_counter() - base
+            if now < arr[i]:
+                time.sleep(arr[i] - now)
+            a, im, vo = bufs(kind, vi, k)
+            plain.run(kind, vi, a, k, im, vo)
+            lat_p.append(time.perf_counter() - base - arr[i])
+        span_p = time.perf_counter() - base
+        # served: a submitting thread at the arrival times, results on the server clock
+        log(f"served open loop lambda={lam:.2f}/s")
+        tickets = [None] * n_req
+        base_s = srv.clock()
+
+        def submitter():
+            for i, (kind, vi, k) in enumerate(reqs):
+                now = srv.clock() - base_s
+                if now < arr[i]:
+                    time.sleep(arr[i] - now)
+                a, im, vo = bufs(kind, vi, k)
+                tickets[i] = (srv.submit_forward(vids[vi], a, im) if kind == "fp"
+                              else srv.submit_back(vids[vi], a, im, vo))
+
+        th = threading.Thread(target=submitter)
+        th.start()
+        th.join()
+        done = [srv.wait(t).t_done - base_s for t in tickets]
+        lat_s = [d - a for d, a in zip(done, arr)]
+        span_s = max(done)
+
+        def pct(x, q):
+            return round(float(np.percentile(np.asarray(x) * 1e3, q)), 2)
+
+        points.append({"lambda_frac_of_plain": frac, "lambda_req_per_s": round(lam, 2),
+                       "plain": {"p50_ms": pct(lat_p, 50), "p99_ms": pct(lat_p, 99),
+                                 "dirs_per_s": round(total_dirs / span_p, 1)},
+                       "served": {"p50_ms": pct(lat_s, 50), "p99_ms": pct(lat_s, 99),
+                                  "dirs_per_s": round(total_dirs / span_s, 1)}})
+    res["poisson"] = points
+    srv.close()
+    plain.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +289,13 @@ def main():
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "bp_kernel_traffic_n128.md"),
                     help="profile summary holding the measured HBM bytes per back-projection direction")
+    ap.add_argument("--serve-requests", type=int, default=150,
+                    help="requests per serving measurement (0: skip the serving leg)")
+    ap.add_argument("--serve-lambdas", default="0.8",
+                    help="Poisson rates as fractions of the plain path's capacity, comma separated")
+    ap.add_argument("--serve-sizes", default="128",
+                    help="volume sizes of the resident volumes (one volume each), comma separated")
+    ap.add_argument("--serve-only", action="store_true", help="run only the serving leg, print its JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -121,6 +308,12 @@ def main():
     from multigpugridder_amd import ops
     from multigpugridder_amd.hostmath import uniform_axes
     from multigpugridder_amd.synth import blob_volume
+
+    if args.serve_only:
+        sizes = [int(x) for x in args.serve_sizes.split(",")]
+        lams = [float(x) for x in args.serve_lambdas.split(",") if x]
+        print(json.dumps(serving_bench(sizes, args.serve_requests, lams)), flush=True)
+        return
 
     dev = torch.device("cuda", 0)
     n = args.n
@@ -229,6 +422,12 @@ def main():
             "avg_launch_ms": round(launch_ms, 4),
         },
     }
+    if args.serve_requests > 0:
+        del cas_imgs, imgs, bp_vol
+        torch.cuda.empty_cache()
+        sizes = [int(x) for x in args.serve_sizes.split(",")]
+        lams = [float(x) for x in args.serve_lambdas.split(",") if x]
+        out["serving"] = serving_bench(sizes, args.serve_requests, lams)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_sample)
     print(json.dumps(out), flush=True)
