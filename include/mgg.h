@@ -182,6 +182,19 @@ int mgg_op_volume_to_cas_volume(const float *volume, int nn, float *cas_volume, 
  * (the exact transpose of mgg_op_volume_to_cas_volume, up to m^3). */
 int mgg_op_cas_volume_to_volume(const float *cas_volume, int m, int pad, float *volume, int nn,
                                 float ker_hwidth, int zero_nyquist, void *stream);
+/* Transposes for differentiation (no counterpart in the reference, which has
+ * no gradients).  The full spatial projector is
+ *   images = CASImgsToImgs(Gather(CASFromVol(volume)))
+ * and its transpose is the composition of these in reverse order with
+ * mgg_op_back_project_adjoint:
+ *   cas_images_to_images_transpose: cas = (1/m^2) ImgsToCAS(images) with the
+ *     index-0 row and column of each CAS image set to 0;
+ *   volume_to_cas_volume_transpose: m^3 CASToVol(cas) keeping the index-0
+ *     planes (= mgg_op_cas_volume_to_volume(zero_nyquist = 0) times m^3). */
+int mgg_op_cas_images_to_images_transpose(const float *images, float *cas_images, int n, int m, int nn,
+                                          void *stream);
+int mgg_op_volume_to_cas_volume_transpose(const float *cas_volume, int m, int pad, float *volume, int nn,
+                                          float ker_hwidth, void *stream);
 /* out[i] = num[i] / (den[i] + eps), evaluated in FP64
  * (MultiGPUGridder_Matlab_Class.m:308). out may alias num. */
 int mgg_op_divide_density(float *out, const float *num, const float *den, size_t n, double eps,

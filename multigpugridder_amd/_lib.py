@@ -97,6 +97,8 @@ _SIGS = {
     "mgg_op_images_to_cas_images": [_vp, _vp, _ci, _ci, _ci, _vp],
     "mgg_op_volume_to_cas_volume": [_vp, _ci, _vp, _ci, _ci, _cf, _vp],
     "mgg_op_cas_volume_to_volume": [_vp, _ci, _ci, _vp, _ci, _cf, _ci, _vp],
+    "mgg_op_cas_images_to_images_transpose": [_vp, _vp, _ci, _ci, _ci, _vp],
+    "mgg_op_volume_to_cas_volume_transpose": [_vp, _ci, _ci, _vp, _ci, _cf, _vp],
     "mgg_op_divide_density": [_vp, _vp, _vp, _sz, _cd, _vp],
     "mgg_op_reduce_partials": [_vp, _vp, _ci, _sz, _sz, _ci, _vp],
     "mgg_op_convert_volume": [_vp, _ci, _vp, _sz, _vp],

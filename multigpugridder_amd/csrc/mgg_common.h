@@ -130,13 +130,16 @@ hipError_t back_project_launch(float *vol, float *dens, const ProjGeom &g, const
                                const float *casImgs, hipStream_t s);
 void chain_cas_to_images(float *casImgs, float2 *herm, float *images, int n, int m, int nn,
                          hipStream_t s);
+// scale / zeroNyq: see launch_herm2d_to_cas (1 / 0 for ImgsToCASImgs itself).
 void chain_images_to_cas(const float *images, float *casImgs, float2 *herm, int n, int m, int nn,
-                         hipStream_t s);
+                         hipStream_t s, float scale = 1.f, int zeroNyq = 0);
 void chain_volume_to_cas(const float *vol, int nn, float *casVol, int m, int pad,
                          const float *dPrecomp, float *real3, float2 *herm3, hipStream_t s);
+// transpose: no 1/m^3 (with zeroNyq = 0 this is the exact transpose of
+// chain_volume_to_cas).
 void chain_cas_to_volume(const float *casVol, int m, int pad, float *vol, int nn,
                          const float *dPrecomp, int zeroNyq, float *real3, float2 *herm3,
-                         hipStream_t s);
+                         hipStream_t s, bool transpose = false);
 void validate_geom(int v, int m, int kerSize, float hw);
 
 }  // namespace mgg

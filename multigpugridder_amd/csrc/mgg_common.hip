@@ -334,13 +334,13 @@ void chain_cas_to_images(float *casImgs, float2 *herm, float *images, int n, int
 
 // images (n x nn^2) -> CAS images (n x m^2); casImgs doubles as the R2C input.
 void chain_images_to_cas(const float *images, float *casImgs, float2 *herm, int n, int m, int nn,
-                         hipStream_t s)
+                         hipStream_t s, float scale, int zeroNyq)
 {
     if (n <= 0) return;
     check_launch(launch_imgs_to_real2d(images, casImgs, m, nn, n, s), "imgs_to_real2d");
     fft2d_batched(0, m, n, casImgs, sizeof(float) * (size_t)m * m, herm,
                   sizeof(float2) * (size_t)m * (m / 2 + 1), s);
-    check_launch(launch_herm2d_to_cas(herm, casImgs, m, n, s), "herm2d_to_cas");
+    check_launch(launch_herm2d_to_cas(herm, casImgs, m, n, s, scale, zeroNyq), "herm2d_to_cas");
 }
 
 void chain_volume_to_cas(const float *vol, int nn, float *casVol, int m, int pad,
@@ -353,12 +353,12 @@ void chain_volume_to_cas(const float *vol, int nn, float *casVol, int m, int pad
 
 void chain_cas_to_volume(const float *casVol, int m, int pad, float *vol, int nn,
                          const float *dPrecomp, int zeroNyq, float *real3, float2 *herm3,
-                         hipStream_t s)
+                         hipStream_t s, bool transpose)
 {
     check_launch(launch_casvol_to_herm3d(casVol, m + 2 * pad, pad, herm3, m, zeroNyq, s),
                  "casvol_to_herm3d");
     fft_exec(3, m, 1, herm3, real3, s);
-    const double scale = 1.0 / ((double)m * (double)m * (double)m);
+    const double scale = transpose ? 1.0 : 1.0 / ((double)m * (double)m * (double)m);
     check_launch(launch_real3d_to_vol(real3, m, vol, nn, dPrecomp, (float)scale, s), "real3d_to_vol");
 }
 

@@ -785,6 +785,39 @@ int mgg_op_images_to_cas_images(const float *images, float *cas_images, int n, i
 }
 
 
+int mgg_op_cas_images_to_images_transpose(const float *images, float *cas_images, int n, int m, int nn,
+                                          void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        need(cas_images, "cas_images"); need(images, "images");
+        if (n < 0 || m <= 0 || nn <= 0 || nn > m || (m % 2) || ((m - nn) % 2))
+            raise(MGG_ERR_INVALID_ARGUMENT, "bad sizes n=%d m=%d nn=%d", n, m, nn);
+        hipStream_t s = as_stream(stream);
+        float2 *herm = scratch_herm(sizeof(float2) * (size_t)std::min<size_t>(n, kFftBatch) * m * (m / 2 + 1), s);
+        const float scale = (float)(1.0 / ((double)m * (double)m));
+        for (int a = 0; a < n; a += (int)kFftBatch) {
+            const int b = std::min<int>((int)kFftBatch, n - a);
+            chain_images_to_cas(images + (size_t)a * nn * nn, cas_images + (size_t)a * m * m, herm, b, m, nn, s,
+                                scale, 1);
+        }
+    });
+}
+
+int mgg_op_volume_to_cas_volume_transpose(const float *cas_volume, int m, int pad, float *volume, int nn,
+                                          float ker_hwidth, void *stream)
+{
+    return guarded([&] {
+        need(volume, "volume"); need(cas_volume, "cas_volume");
+        if (nn <= 0 || m < nn || (m % 2) || ((m - nn) % 2) || pad < 0)
+            raise(MGG_ERR_INVALID_ARGUMENT, "bad sizes nn=%d m=%d pad=%d", nn, m, pad);
+        hipStream_t s = as_stream(stream);
+        Vol3Scratch &sc = vol_scratch(m, (int)lroundf(ker_hwidth), s);
+        chain_cas_to_volume(cas_volume, m, pad, volume, nn, sc.precomp.as<float>(), 0, sc.real.as<float>(),
+                            sc.herm.as<float2>(), s, true);
+    });
+}
+
 int mgg_op_volume_to_cas_volume(const float *volume, int nn, float *cas_volume, int m, int pad, float ker_hwidth,
                                 void *stream)
 {

@@ -57,8 +57,10 @@ hipError_t launch_real2d_to_imgs(const float *real, float *imgs, int M, int N, i
 // images (n x N x N) -> zero-padded, sign-modulated real input (n x M x M).
 hipError_t launch_imgs_to_real2d(const float *imgs, float *real, int M, int N, int n,
                                  hipStream_t s);
-// R2C output (n x M x (M/2+1)) -> CAS images (n x M x M).
-hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hipStream_t s);
+// R2C output (n x M x (M/2+1)) -> CAS images (n x M x M), times scale; with
+// zeroNyq the index-0 row and column of every CAS image are written as 0.
+hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hipStream_t s,
+                                float scale = 1.f, int zeroNyq = 0);
 
 // ---- volume chains (3-D) -----------------------------------------------
 // volume (N^3) -> padded, precompensated, sign-modulated real input (M^3).

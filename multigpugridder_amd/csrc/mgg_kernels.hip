@@ -984,7 +984,7 @@ __global__ void k_imgs_to_real2d(const float *__restrict__ imgs, float *__restri
 
 // fftshift + ComplexToCAS (gpuGridder.cpp:584-595) from the R2C half spectrum.
 __global__ void k_herm2d_to_cas(const float2 *__restrict__ herm, float *__restrict__ cas, int M,
-                                size_t total)
+                                size_t total, float scale, int zeroNyq)
 {
     const int H = M / 2 + 1;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
@@ -1003,7 +1003,9 @@ __global__ void k_herm2d_to_cas(const float2 *__restrict__ herm, float *__restri
             re = f.x; im = -f.y;
         }
         const float s = chk(cx + cy);
-        cas[e] = s * re + s * im;
+        // zeroNyq: the index-0 row / column FromCAS discards (transpose of the
+        // inverse image chain); scale is exact (1) on the forward chain.
+        cas[e] = (zeroNyq && (cx == 0 || cy == 0)) ? 0.f : (s * re + s * im) * scale;
     }
 }
 
@@ -1535,11 +1537,13 @@ hipError_t launch_imgs_to_real2d(const float *imgs, float *real, int M, int N, i
     return hipGetLastError();
 }
 
-hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hipStream_t s)
+hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hipStream_t s, float scale,
+                                int zeroNyq)
 {
     const size_t total = (size_t)n * M * M;
     if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_herm2d_to_cas, dim3(grid_for(total)), dim3(kThreads), 0, s, herm, cas, M, total);
+    hipLaunchKernelGGL(k_herm2d_to_cas, dim3(grid_for(total)), dim3(kThreads), 0, s, herm, cas, M, total, scale,
+                       zeroNyq);
     return hipGetLastError();
 }
 

@@ -172,6 +172,34 @@ def cas_volume_to_volume(cas_volume: torch.Tensor, n: int, interp_factor: float 
     return out
 
 
+def cas_images_to_images_transpose(images: torch.Tensor, m: int,
+                                   out: torch.Tensor | None = None) -> torch.Tensor:
+    """Transpose of :func:`cas_images_to_images`: (P, n, n) -> (P, m, m),
+    (1/m^2) ImgsToCAS with the index-0 row / column of each CAS image zeroed."""
+    _check(images, "images")
+    p, n = images.shape[0], images.shape[-1]
+    if out is None:
+        out = torch.empty((p, m, m), device=images.device, dtype=torch.float32)
+    _check(out, "out")
+    _lib.call("mgg_op_cas_images_to_images_transpose", _ptr(images), _ptr(out), p, m, n, _stream())
+    return out
+
+
+def volume_to_cas_volume_transpose(cas_volume: torch.Tensor, n: int, interp_factor: float = 2.0,
+                                   extra_padding: int = 3, hwidth: float = 2.0,
+                                   out: torch.Tensor | None = None) -> torch.Tensor:
+    """Transpose of :func:`volume_to_cas_volume`: (V^3) -> (n^3), m^3 CASToVol
+    keeping the index-0 planes."""
+    _check(cas_volume, "cas_volume")
+    m = int(np.float32(n) * np.float32(interp_factor))
+    if out is None:
+        out = torch.empty((n, n, n), device=cas_volume.device, dtype=torch.float32)
+    _check(out, "out")
+    _lib.call("mgg_op_volume_to_cas_volume_transpose", _ptr(cas_volume), m, extra_padding, _ptr(out), n,
+              float(hwidth), _stream())
+    return out
+
+
 def divide_density(num: torch.Tensor, den: torch.Tensor, eps: float = 1e-6,
                    out: torch.Tensor | None = None) -> torch.Tensor:
     _check(num, "num")

@@ -64,6 +64,8 @@ void orc_forward_project(const float *vol, int V, const float *axes, int nAxes,
     const float kerScale = kerCenter / kerHWidth;
     const long long V3 = (long long)V * V * V;
 
+    /* directions are independent: one thread per direction, same sums */
+#pragma omp parallel for schedule(dynamic, 1)
     for (int a = 0; a < nAxes; a++) {
         const float *nx = axes + 9 * a;
         const float *ny = nx + 3;

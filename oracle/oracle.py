@@ -335,6 +335,41 @@ def imgs_to_cas_imgs(imgs, m):
     return to_cas(f)
 
 
+def cas_imgs_to_imgs_transpose(imgs, m):
+    """Transpose of :func:`cas_imgs_to_imgs`: (1/m^2) * imgs_to_cas_imgs with
+    the index-0 row and column of each CAS image zeroed (FromCAS discards
+    them).  Pinned against the explicit matrix in the tests."""
+    c = imgs_to_cas_imgs(imgs, m) / float(m * m)
+    c[:, 0, :] = 0.0
+    c[:, :, 0] = 0.0
+    return c
+
+
+def vol_to_cas_transpose(cas_vol, n, interp_factor=2, extra_padding=3, hw=2):
+    """Transpose of :func:`vol_to_cas`: m^3 * cas_to_vol keeping the index-0
+    planes.  Pinned against the explicit matrix in the tests."""
+    m = int(round(n * interp_factor))
+    return float(m) ** 3 * cas_to_vol(cas_vol, n, interp_factor, extra_padding, hw, zero_nyquist=False)
+
+
+def projector(vol, axes, ker, interp_factor=2, extra_padding=3):
+    """The full spatial projector images = CASImgsToImgs(Gather(CASFromVol(vol))),
+    FP64 except the FP32 CAS volume the gather reads (as the GPU path)."""
+    n = np.asarray(vol).shape[0]
+    m = int(round(n * interp_factor))
+    cas = vol_to_cas(vol, interp_factor, extra_padding).astype(np.float32)
+    return cas_imgs_to_imgs(forward_project(cas, axes, ker, m), n)
+
+
+def projector_transpose(imgs, axes, ker, n, interp_factor=2, extra_padding=3):
+    """Exact transpose of :func:`projector` (FP64), composed stage by stage."""
+    m = int(round(n * interp_factor))
+    v = m + 2 * extra_padding
+    c = cas_imgs_to_imgs_transpose(imgs, m)
+    vol_c = back_project_adjoint(c.astype(np.float32), axes, ker, v)
+    return vol_to_cas_transpose(vol_c, n, interp_factor, extra_padding)
+
+
 def reconstruct(cas_vol_bp, cas_vol_density, n, interp_factor=2, extra_padding=3, hw=2):
     """MultiGPUGridder_Matlab_Class.m:305-309: CAS(BP) / (CAS(density)+1e-6)
     then CAS -> volume."""

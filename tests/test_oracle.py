@@ -348,3 +348,47 @@ def test_golden_fixtures_reproduce(oracle, ker32):
     np.testing.assert_allclose(bp, g["bp"], rtol=0, atol=1e-12)
     adj = oracle.back_project_adjoint(g["cas_imgs"], g["axes"], g["ker"], int(g["v"]))
     np.testing.assert_allclose(adj, g["adj"], rtol=0, atol=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# Transposes used by the differentiable projector, pinned by explicit matrices
+# ---------------------------------------------------------------------------
+def _matrix(fn, in_shape):
+    cols = []
+    for k in range(int(np.prod(in_shape))):
+        e = np.zeros(int(np.prod(in_shape)))
+        e[k] = 1.0
+        cols.append(np.asarray(fn(e.reshape(in_shape)), dtype=np.float64).ravel())
+    return np.stack(cols, axis=1)
+
+
+def test_image_chain_transpose_is_exact(oracle):
+    n, m = 4, 8
+    A = _matrix(lambda c: oracle.cas_imgs_to_imgs(c, n), (1, m, m))        # (n^2, m^2)
+    T = _matrix(lambda y: oracle.cas_imgs_to_imgs_transpose(y, m), (1, n, n))  # (m^2, n^2)
+    assert np.max(np.abs(A.T - T)) < 1e-12
+    # without clearing the index-0 row / column the candidate is not the transpose
+    T0 = _matrix(lambda y: oracle.imgs_to_cas_imgs(y, m) / m ** 2, (1, n, n))
+    assert np.max(np.abs(A.T - T0)) > 1e-3
+
+
+def test_volume_chain_transpose_is_exact(oracle):
+    n, m, pad = 4, 8, 1
+    G = _matrix(lambda x: oracle.vol_to_cas(x, 2, pad), (n, n, n))               # (V^3, n^3)
+    T = _matrix(lambda c: oracle.vol_to_cas_transpose(c, n, 2, pad), (m + 2 * pad,) * 3)
+    assert np.max(np.abs(G.T - T)) < 1e-9 * np.max(np.abs(G))
+    # FromCAS's zeroing of the index-0 planes would break it
+    T0 = _matrix(lambda c: m ** 3 * oracle.cas_to_vol(c, n, 2, pad, zero_nyquist=True), (m + 2 * pad,) * 3)
+    assert np.max(np.abs(G.T - T0)) > 1e-3 * np.max(np.abs(G))
+
+
+def test_projector_adjoint_identity(oracle, ker32):
+    n, p = 8, 9
+    from multigpugridder_amd.synth import random_directions
+    axes = random_directions(p, 21)
+    rng = np.random.default_rng(4)
+    v = rng.standard_normal((n, n, n))
+    y = rng.standard_normal((p, n, n))
+    lhs = np.sum(oracle.projector(v, axes, ker32) * y)
+    rhs = np.sum(v * oracle.projector_transpose(y, axes, ker32, n))
+    assert abs(lhs - rhs) / max(abs(lhs), abs(rhs)) < 1e-6
