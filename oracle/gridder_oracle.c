@@ -164,6 +164,8 @@ void orc_back_project(double *volOut, int V, const float *axes, int nAxes,
     const float kerScale = kerCenter / kerHWidth;
     const long long M2 = (long long)M * M;
 
+    /* each voxel is owned by one thread and sums its directions in order */
+#pragma omp parallel for schedule(dynamic, 1)
     for (int vk = 0; vk < V; vk++)
     for (int vj = 0; vj < V; vj++)
     for (int vi = 0; vi < V; vi++) {
