@@ -235,8 +235,11 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
 // accumulation rounding differs.
 // ---------------------------------------------------------------------------
 constexpr int FPT = 8;          // tile edge (pixels)
-constexpr int FPBOX = 3072;     // per-wave staged box budget (floats)
-constexpr int FPKER = 1152;     // table floats kept in LDS (padded 501-entry table fits)
+// Per-wave staged box budget (floats).  Four boxes plus the padded table take
+// 53,760 bytes of LDS, which keeps three workgroups on a CU; a larger budget
+// loses the third workgroup and is slower overall.
+constexpr int FPBOX = 3072;     // per-wave staged box budget (floats): 4 boxes + table = 54,272 B, 3 workgroups per CU
+constexpr int FPKER = 1152;     // table floats kept in LDS (the padded 501-entry table fits)      // table floats kept in LDS (the padded 501-entry table fits)
 
 __device__ __forceinline__ void wave_lds_sync_fp()
 {
@@ -245,6 +248,10 @@ __device__ __forceinline__ void wave_lds_sync_fp()
 }
 
 // Padded 501-entry table lookup (see bp_fast_kernel for the exactness argument).
+// The forward gather evaluates r = (X - 2 + t) - x with X = round(x), so
+// |r| <= 2.5 and the index stays within [-63, 563], inside the padding: no
+// clamp is needed.  (A non-finite coordinate gives an index the LDS bounds
+// check turns into a read of 0 or of another LDS array, never a fault.)
 constexpr int FKPAD = 320;
 __device__ __forceinline__ float kb_fast_fp(const float *__restrict__ sk, float r)
 {
@@ -252,7 +259,6 @@ __device__ __forceinline__ float kb_fast_fp(const float *__restrict__ sk, float 
     s = s + 250.0f;
     int i;
     asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(s));
-    i = min(max(i, -FKPAD), 500 + FKPAD);
     return sk[FKPAD + i];
 }
 
@@ -332,7 +338,17 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     const int Dy = __builtin_amdgcn_readfirstlane(hi(u1, v1)) - by0 + 1;
     const int Dz = __builtin_amdgcn_readfirstlane(hi(u2, v2)) - bz0 + 1;
     const int segs = (Dx + 3) >> 2;  // 16-byte segments per box row
-    const int RP = segs * 4;         // LDS row pitch (floats), 16-byte aligned
+    // LDS pitches (floats), multiples of 4 (16-byte rows).  When the box
+    // still fits, the row and plane pitches are padded to an odd number of
+    // 16-byte units so that the rows and planes a tile's pixels read spread
+    // over the banks (a pitch of 16 puts every row at the same bank offset);
+    // otherwise the rows are packed.
+    int RP = (segs | 1) * 4;
+    int PP = RP * Dy + ((Dy & 1) ? 0 : RP);
+    if (PP * Dz > FPBOX) {
+        RP = segs * 4;
+        PP = RP * Dy;
+    }
     const long long V3 = (long long)V * V * V;
     const T *vol = (const T *)p.vol;
 
@@ -356,56 +372,82 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         for (int bb = 0; bb < 5; bb++) wij[aa][bb] = wi[aa] * wj[bb];
 
     float acc = 0.f;
-    if (Dx >= 5 && Dy >= 5 && Dz >= 5 && Dx <= 64 && Dy <= 64 && Dz <= 64 && RP * Dy * Dz <= FPBOX &&
-        V3 < (1LL << 31)) {
+    if (Dx >= 5 && Dy >= 5 && Dz >= 5 && Dx <= 64 && Dy <= 64 && Dz <= 64 && PP * Dz <= FPBOX &&
+        V3 * (long long)sizeof(T) < (1LL << 32)) {
         // Stage: lane -> (row slot, segment); 64 / segs rows per pass.
         const int rowsPer = 64 / segs;
         const int rIn = ln / segs, sg = ln - rIn * segs;
         const int nRows = Dy * Dz;
         const int VV = V * V, V3i = (int)V3;
-        const float invDy = 1.0f / (float)Dy;
+        // row -> (ez, ey) = (row / Dy, row % Dy), stepped by rowsPer per pass
+        const int qs = rowsPer / Dy, rs = rowsPer - qs * Dy;
+        const int ez = rIn / Dy, ey = rIn - ez * Dy;
+        const int lin = (bz0 + ez) * VV + (by0 + ey) * V + (bx0 + 4 * sg);
+        const int linStep = qs * VV + rs * V, linWrap = VV - Dy * V;
+        const int dStep = qs * PP + rs * RP, dWrap = PP - Dy * RP;
+        // The reference bounds only the linear index (gpuForwardProject.cu:90):
+        // when the box's whole index range is inside the volume no element
+        // needs a check.  lin grows with ez, ey and the segment, so the range
+        // is spanned by the first and last segment of the box.
+        const long long linLo = (long long)bz0 * VV + (long long)by0 * V + bx0;
+        const long long linHi = (long long)(bz0 + Dz - 1) * VV + (long long)(by0 + Dy - 1) * V + bx0 + RP - 1;
+        const bool boxIn = linLo >= 0 && linHi < V3;
         // Four passes per batch: all loads of a batch are in flight before
-        // the first LDS store waits on them.
+        // the first LDS store waits on them.  The checked and unchecked
+        // variants are separate loops (boxIn is wave-uniform).
         constexpr int SB = 4;
-        for (int r0 = 0; r0 < nRows; r0 += SB * rowsPer) {
-            float v4[SB][4];
-            int dst[SB];
+        auto stage = [&](auto inBox) {
+            int row = rIn, e_y = ey, l = lin;
+            int d = ez * PP + ey * RP + 4 * sg;  // LDS offset of the lane's segment
+            for (int r0 = 0; r0 < nRows; r0 += SB * rowsPer) {
+                float v4[SB][4];
+                int dst[SB];
 #pragma unroll
-            for (int q = 0; q < SB; q++) {
-                const int row = r0 + q * rowsPer + rIn;
-                dst[q] = -1;
-                if (rIn < rowsPer && row < nRows) {
-                    // row / Dy: (row + 0.5) / Dy is at least 1/(2 Dy) from an integer
-                    const int ez = (int)(((float)row + 0.5f) * invDy), ey = row - ez * Dy;
-                    const int lin = (bz0 + ez) * VV + (by0 + ey) * V + (bx0 + 4 * sg);
-                    if (lin >= 0 && lin + 3 < V3i) {
-                        load4<L>(vol + lin, v4[q]);
-                    } else {
-                        // the reference's bound on the linear index (gpuForwardProject.cu:90)
+                for (int q = 0; q < SB; q++) {
+                    dst[q] = -1;
+                    if (rIn < rowsPer && row < nRows) {
+                        if (decltype(inBox)::value) {
+                            load4<L>(vol + l, v4[q]);
+                        } else {
 #pragma unroll
-                        for (int e = 0; e < 4; e++)
-                            v4[q][e] = (lin + e >= 0 && lin + e < V3i) ? L::one(vol + lin + e) : 0.f;
+                            for (int e = 0; e < 4; e++)
+                                v4[q][e] = (l + e >= 0 && l + e < V3i) ? L::one(vol + l + e) : 0.f;
+                        }
+                        dst[q] = d;
                     }
-                    dst[q] = row * RP + 4 * sg;
+                    row += rowsPer;
+                    e_y += rs;
+                    l += linStep;
+                    d += dStep;
+                    if (e_y >= Dy) {
+                        e_y -= Dy;
+                        l += linWrap;
+                        d += dWrap;
+                    }
                 }
-            }
 #pragma unroll
-            for (int q = 0; q < SB; q++) {
-                if (dst[q] >= 0) {
-                    f4 w4 = {v4[q][0], v4[q][1], v4[q][2], v4[q][3]};
-                    *(f4 *)(box + dst[q]) = w4;
+                for (int q = 0; q < SB; q++) {
+                    if (dst[q] >= 0) {
+                        f4 w4 = {v4[q][0], v4[q][1], v4[q][2], v4[q][3]};
+                        *(f4 *)(box + dst[q]) = w4;
+                    }
                 }
             }
+        };
+        if (boxIn) {
+            stage(std::true_type{});
+        } else {
+            stage(std::false_type{});
         }
         wave_lds_sync_fp();
         if (active) {
-            const float *bp = box + ((Z - 2 - bz0) * Dy + (Y - 2 - by0)) * RP + (X - 2 - bx0);
+            const float *bp = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (X - 2 - bx0);
 #pragma unroll
             for (int cc = 0; cc < 5; cc++) {
                 float sc = 0.f;
 #pragma unroll
                 for (int bb = 0; bb < 5; bb++) {
-                    const float *rp = bp + (cc * Dy + bb) * RP;
+                    const float *rp = bp + cc * PP + bb * RP;
 #pragma unroll
                     for (int aa = 0; aa < 5; aa++) sc = fmaf(wij[aa][bb], rp[aa], sc);
                 }
@@ -442,7 +484,7 @@ __global__ __launch_bounds__(kThreads) void fp_lds_kernel(FPArgs p)
         for (int t = threadIdx.x; t < p.K; t += kThreads) sker[t] = p.ker[t];
     }
     __syncthreads();
-    // Persistent: each wave walks work items wv + 4*blockIdx.x, + 4*gridDim.x, ...
+    // Each wave takes work items wv + 4*blockIdx.x, + 4*gridDim.x, ...
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
         fp_lds_tile<DT, FAST>(p, sker, sbox[wv], work, ln);
@@ -710,6 +752,16 @@ __device__ __forceinline__ float kb_fast(const float *__restrict__ sk, float r)
     return sk[KPAD + i];
 }
 
+// Same lookup without the clamp, for directions whose frame is orthonormal
+// (checked per chunk): then every tap has |r| <= 4.13 and the index stays in
+// [-267, 767], inside the padding (see bp_fast_kernel).
+__device__ __forceinline__ float kb_fast_nc(const float *__restrict__ sk, float r)
+{
+    float s = r * 125.0f;
+    s = s + 250.0f;
+    return sk[KPAD + cvt_rpi(s)];
+}
+
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -723,6 +775,7 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
     __shared__ float sax[BPCH * 9];
     __shared__ int sidx[BPCH];
     __shared__ int swave[kThreads / 64];
+    __shared__ int snon[kThreads / 64];
     __shared__ float sacc[4][BV];
     __shared__ float sden[DENS ? 4 : 1][DENS ? BV : 1];
     __shared__ unsigned short squeue[4][QCAP];
@@ -770,7 +823,7 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
     for (int base = 0; base < p.nAxes; base += BPCH) {
         const int n = min(BPCH, p.nAxes - base);
         __syncthreads();
-        bool pass = false;
+        bool pass = false, orth = true;
         float a9[9];
         if (t < n) {
             const float *ax = p.axes + (size_t)9 * (base + t);
@@ -779,10 +832,24 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
             const float d = bcx * a9[6] + bcy * a9[7] + bcz * a9[8];
             const float sup = bc * (fabsf(a9[6]) + fabsf(a9[7]) + fabsf(a9[8]));
             pass = fabsf(d) <= p.hw + sup + 1e-3f;
+            // [u v n] orthonormal to 1e-4 (rotation matrices are, to ~1e-7)
+            const float uu = a9[0] * a9[0] + a9[1] * a9[1] + a9[2] * a9[2];
+            const float vv = a9[3] * a9[3] + a9[4] * a9[4] + a9[5] * a9[5];
+            const float nn = a9[6] * a9[6] + a9[7] * a9[7] + a9[8] * a9[8];
+            const float uv = a9[0] * a9[3] + a9[1] * a9[4] + a9[2] * a9[5];
+            const float un = a9[0] * a9[6] + a9[1] * a9[7] + a9[2] * a9[8];
+            const float vn = a9[3] * a9[6] + a9[4] * a9[7] + a9[5] * a9[8];
+            orth = fabsf(uu - 1.f) <= 1e-4f && fabsf(vv - 1.f) <= 1e-4f && fabsf(nn - 1.f) <= 1e-4f &&
+                   fabsf(uv) <= 1e-4f && fabsf(un) <= 1e-4f && fabsf(vn) <= 1e-4f;
         }
         const unsigned long long bal = __ballot(pass);
-        if (ln == 0) swave[wv] = __popcll(bal);
+        const unsigned long long nonOrth = __ballot(!orth);
+        if (ln == 0) {
+            swave[wv] = __popcll(bal);
+            snon[wv] = nonOrth != 0ull;
+        }
         __syncthreads();
+        const bool chunkOrtho = !(snon[0] | snon[1] | snon[2] | snon[3]);
         int off = 0, total = 0;
 #pragma unroll
         for (int w = 0; w < kThreads / 64; w++) {
@@ -823,6 +890,7 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
             const bool last = q + 4 >= total;
             wave_lds_sync();
             // Gather full groups (and, after the last direction, the rest).
+            auto gather = [&](auto clampT) {
             while (tail - head >= 64 || (last && tail > head)) {
                 const int e = head + ln;
                 head = min(head + 64, tail);
@@ -873,9 +941,18 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                             const float px = Ax[ii] + Bx + cvf;
                             const float py = Ay[ii] + By + cvf;
                             const float pz = Az[ii] + Bz + cvf;
-                            float w = kb_fast(sker, px - fvi);
-                            w = w * kb_fast(sker, py - fvj);
-                            w = w * kb_fast(sker, pz - fvk);
+                            float w, wy, wz;
+                            if (decltype(clampT)::value) {
+                                w = kb_fast(sker, px - fvi);
+                                wy = kb_fast(sker, py - fvj);
+                                wz = kb_fast(sker, pz - fvk);
+                            } else {
+                                w = kb_fast_nc(sker, px - fvi);
+                                wy = kb_fast_nc(sker, py - fvj);
+                                wz = kb_fast_nc(sker, pz - fvk);
+                            }
+                            w = w * wy;
+                            w = w * wz;
                             cum = fmaf(val[ii], w, cum);
                             if (DENS) {
                                 const int lin = row + ii;
@@ -889,6 +966,11 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                     if (DENS) atomicAdd(&sden[wv][vox], dsum);
                 }
             }
+            };
+            if (chunkOrtho)
+                gather(std::false_type{});
+            else
+                gather(std::true_type{});
             wave_lds_sync();  // queue slots consumed before they are refilled
             if (last) break;
         }
@@ -1374,15 +1456,8 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
         if (work > 0xffffffffull / 2) return hipErrorInvalidValue;
         a.totalWork = (unsigned)work;
         a.perXcd = 0;
-        // persistent grid: a few resident workgroups per CU
-        static int cus = 0;
-        if (!cus) {
-            int dev = 0;
-            if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0)
-                cus = 256;
-        }
-        const unsigned groups = (a.totalWork + 3) / 4;
-        const dim3 grid(std::min<unsigned>(groups, (unsigned)cus * 3u)), block(kThreads);
+        // one workgroup per four tiles (a persistent grid measured slower)
+        const dim3 grid((a.totalWork + 3) / 4), block(kThreads);
 #define MGG_FP_LAUNCH(DTV)                                                                   \
     if (fast) hipLaunchKernelGGL((fp_lds_kernel<DTV, true>), grid, block, 0, s, a);          \
     else hipLaunchKernelGGL((fp_lds_kernel<DTV, false>), grid, block, 0, s, a);
