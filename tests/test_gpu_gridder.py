@@ -147,3 +147,58 @@ def test_legacy_ctypes_symbols(cuda, oracle, ker32):
     vol = np.ctypeslib.as_array(lib.GetVolume(h), shape=(v, v, v))
     assert rel_l2(vol, oracle.back_project(out, axes, ker32, v, mask_radius=m / 2 - 2)) < TOL
     lib.mgg_destroy(h)
+
+
+def test_config_c1_n64_500_random_directions(cuda, oracle, ker32):
+    """The CPU-runnable configuration end to end through the gridder object:
+    blob volume 64^3 (seed 1), interp 2, 500 random-quaternion directions
+    (seed 100); forward projection to 500 images, then back projection of
+    them into a zeroed volume and getVol, all against the FP64 oracle."""
+    from multigpugridder_amd import MultiGPUGridder, blob_volume, random_directions
+
+    n, p = 64, 500
+    m, v = 2 * n, 2 * n + 6
+    print(f"config C1 n={n} p={p}", flush=True)
+    vol = blob_volume(n, 1)
+    axes = random_directions(p, 100)
+    g = MultiGPUGridder(n, p, 2)
+    g.setVolume(vol)
+    imgs = g.forwardProject(axes)
+    cas, ref = _oracle_forward(oracle, ker32, vol, axes, n)
+    assert rel_l2(imgs, ref) < TOL
+    g.resetVolume()
+    g.backProject(imgs, axes)
+    got = g.getVol()
+    bp = oracle.back_project(oracle.imgs_to_cas_imgs(imgs, m).astype(np.float32), axes, ker32, v)
+    assert rel_l2(got, oracle.cas_to_vol(bp, n)) < TOL
+    g.delete()
+
+
+def test_config_c2_n128_grid_subset(cuda, oracle, ker32):
+    """The plain drop-in configuration (blob volume 128^3, interp 2, the
+    100 x 100 uniform grid) on a fixed 200-direction subset (every 50th grid
+    direction): forwardProject, backProject and reconstructVol against the FP64
+    oracle."""
+    from multigpugridder_amd import MultiGPUGridder, blob_volume
+
+    n = 128
+    m, v = 2 * n, 2 * n + 6
+    axes = oracle.uniform_axes(100, 100)[::50].copy()
+    p = axes.shape[0]
+    print(f"config C2 n={n} p={p}", flush=True)
+    vol = blob_volume(n, 1)
+    g = MultiGPUGridder(n, p, 2)
+    g.setVolume(vol)
+    imgs = g.forwardProject(axes)
+    cas, ref = _oracle_forward(oracle, ker32, vol, axes, n)
+    assert rel_l2(imgs, ref) < TOL
+    g.resetVolume()
+    g.backProject(imgs, axes)
+    cas_vol = g.casVolumeToHost()
+    cas_imgs = oracle.imgs_to_cas_imgs(imgs, m).astype(np.float32)
+    bp = oracle.back_project(cas_imgs, axes, ker32, v)
+    assert rel_l2(cas_vol, bp) < TOL
+    rec = g.reconstructVol(imgs, axes)
+    dens = oracle.back_project(np.ones_like(cas_imgs), axes, ker32, v)
+    assert rel_l2(rec, oracle.reconstruct(bp, dens, n)) < 1e-4
+    g.delete()
