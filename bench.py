@@ -282,7 +282,8 @@ def main():
     ap.add_argument("--ntheta", type=int, default=100)
     ap.add_argument("--nphi", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=1000, help="directions per launch")
-    ap.add_argument("--cpu-sample", type=int, default=400)
+    ap.add_argument("--cpu-sample", type=int, default=2000,
+                    help="grid directions the CPU baseline runs (about 15 s of CPU work on 16 threads)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size "

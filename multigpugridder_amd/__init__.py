@@ -9,6 +9,7 @@ from ._lib import LibraryNotBuilt, MggError, lib  # noqa: F401
 from .gridder import MultiGPUGridder  # noqa: F401
 from .server import Server, pinned_empty  # noqa: F401
 from .autograd import ForwardProjection, project, project_transpose  # noqa: F401
+from .mrc import read_mrc, write_mrc  # noqa: F401
 from .hostmath import kb_table, precomp, uniform_axes  # noqa: F401
 from .synth import blob_volume, random_directions  # noqa: F401
 

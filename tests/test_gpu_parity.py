@@ -120,6 +120,48 @@ def test_back_project_reference(cuda, oracle, ker32, n, p, seed):
     assert_parity(_np(dens), ref_d)
 
 
+@pytest.mark.parametrize("scale", [1.05, 0.9])
+def test_back_project_non_orthonormal_axes(cuda, oracle, ker32, scale):
+    """Direction frames that are not orthonormal (scaled rotations, and one
+    chunk mixing them with exact rotations) take the clamped table lookup;
+    results still follow the reference arithmetic."""
+    import torch
+
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.synth import random_directions
+
+    n, p = 24, 150
+    m, v = 2 * n, 2 * n + 6
+    axes = random_directions(p, 31)
+    axes[::3] *= np.float32(scale)
+    print(f"BP non-orthonormal scale={scale}", flush=True)
+    y = np.random.default_rng(5).standard_normal((p, m, m)).astype(np.float32)
+    out = torch.zeros((v, v, v), device=cuda)
+    ops.back_project(_t(y, cuda), _t(axes, cuda), v, ker=_t(ker32, cuda), out=out)
+    assert_parity(_np(out), oracle.back_project(y, axes, ker32, v))
+    img = ops.forward_project(_t(oracle.vol_to_cas(np.ones((n, n, n)), 2, 3).astype(np.float32), cuda),
+                              _t(axes, cuda), m, ker=_t(ker32, cuda))
+    cas = oracle.vol_to_cas(np.ones((n, n, n)), 2, 3).astype(np.float32)
+    assert_parity(_np(img), oracle.forward_project(cas, axes, ker32, m))
+
+
+def test_zero_directions_are_no_ops(cuda):
+    import torch
+
+    from multigpugridder_amd import ops
+
+    v, m = 38, 32
+    cas = torch.ones((v, v, v), device=cuda)
+    axes = torch.zeros((0, 9), device=cuda)
+    imgs = ops.forward_project(cas, axes, m)
+    assert tuple(imgs.shape) == (0, m, m)
+    out = torch.full((v, v, v), 3.0, device=cuda)
+    ops.back_project(torch.zeros((0, m, m), device=cuda), axes, v, out=out)
+    ops.back_project_adjoint(torch.zeros((0, m, m), device=cuda), axes, v, out=out)
+    torch.cuda.synchronize()
+    assert bool((out == 3.0).all())
+
+
 def test_back_project_repeatable(cuda, oracle, ker32):
     """Two identical back projections give bit-identical volumes."""
     import torch
