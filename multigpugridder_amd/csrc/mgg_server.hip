@@ -516,10 +516,13 @@ struct mgg_server {
             std::unique_ptr<Batch> b = form_batch();
             if (!b) {
                 if (inflight.empty()) {
-                    // Nothing will free memory: the head cannot run at all.
+                    // Nothing will free memory: the head cannot run at all.  Its
+                    // unissued directions are abandoned (every issued batch has
+                    // finished, since nothing is in flight).
                     Req *h = queue.front();
                     queue.pop_front();
                     h->issued = h->r.n_axes;
+                    h->done = h->r.n_axes;
                     fail_req(*h, MGG_ERR_OUT_OF_MEMORY,
                              "request does not fit in the server's device-memory budget");
                 } else {

@@ -213,6 +213,28 @@ def test_server_admission_by_budget(cuda):
             s.wait(999)
 
 
+def test_server_fails_request_that_no_longer_fits(cuda):
+    """A queued request whose batch cannot fit once more volumes are resident
+    fails with MGG_ERR_OUT_OF_MEMORY (its wait returns) instead of hanging."""
+    from multigpugridder_amd import MggError, Server, blob_volume, random_directions
+    from multigpugridder_amd import _lib
+
+    n = 64
+    # 49 MiB: the N=64 volume (9.6 MB) leaves 41.8 MB, enough for one batch of 256
+    # directions (38.0 MB: CAS images, half spectra, images); an N=48 volume
+    # (4.2 MB resident) then leaves 37.5 MB, which is not
+    with Server(budget_bytes=49 << 20, max_batch_directions=256) as s:
+        vid = s.add_volume(blob_volume(n, 1))
+        s.hold(True)
+        out = np.empty((300, n, n), np.float32)
+        t = s.submit_forward(vid, random_directions(300, 2), out)
+        s.add_volume(blob_volume(48, 2))
+        s.hold(False)
+        with pytest.raises(MggError) as e:
+            s.wait(t, timeout=60.0)
+        assert e.value.code == _lib.MGG_ERR_OUT_OF_MEMORY
+
+
 def test_server_wait_timeout_then_result(cuda):
     from multigpugridder_amd import MggError, Server, blob_volume, pinned_empty, random_directions
     from multigpugridder_amd import _lib
