@@ -282,6 +282,8 @@ def main():
     ap.add_argument("--ntheta", type=int, default=100)
     ap.add_argument("--nphi", type=int, default=100)
     ap.add_argument("--chunk", type=int, default=1000, help="directions per launch")
+    ap.add_argument("--streams", type=int, default=2,
+                    help="streams the chunks of a step are spread over (image chains overlap projectors)")
     ap.add_argument("--cpu-sample", type=int, default=2000,
                     help="grid directions the CPU baseline runs (about 15 s of CPU work on 16 threads)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -328,37 +330,68 @@ def main():
     axes = torch.from_numpy(axes_np).to(dev)
     ker = ops.kb_table(dev)
     cas_vol = ops.volume_to_cas_volume(vol)
-    cas_imgs = torch.empty((chunk, m, m), device=dev)
+    nst = max(1, args.streams)
+    streams = [torch.cuda.Stream(device=dev) for _ in range(nst)]  # default priority
+    cas_bufs = [torch.empty((chunk, m, m), device=dev) for _ in range(nst)]
     imgs = torch.empty((p, n, n), device=dev)
     bp_vol = torch.zeros((v, v, v), device=dev)
     torch.cuda.synchronize()
 
     ev = {"fp": [], "bp": [], "fp_chain": [], "bp_chain": []}
 
-    def timed(name, fn, record):
-        if record:
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
-            a.record()
-            fn()
-            b.record()
-            ev[name].append((a, b))
-        else:
-            fn()
+    def timed(name, fn, record, st):
+        # HIP events on the stream the kernels run on
+        with torch.cuda.stream(st):
+            if record:
+                a = torch.cuda.Event(enable_timing=True)
+                b = torch.cuda.Event(enable_timing=True)
+                a.record(st)
+                fn()
+                b.record(st)
+                ev[name].append((a, b))
+            else:
+                fn()
 
     def step(record):
-        for s in range(0, p, chunk):
-            c = min(chunk, p - s)
-            ax = axes[s:s + c]
-            timed("fp", lambda: ops.forward_project(cas_vol, ax, m, ker=ker, out=cas_imgs[:c]), record)
-            timed("fp_chain", lambda: ops.cas_images_to_images(cas_imgs[:c], n, out=imgs[s:s + c], clobber=True),
-                  record)
-        bp_vol.zero_()
-        for s in range(0, p, chunk):
-            c = min(chunk, p - s)
-            ax = axes[s:s + c]
-            timed("bp_chain", lambda: ops.images_to_cas_images(imgs[s:s + c], m, out=cas_imgs[:c]), record)
-            timed("bp", lambda: ops.back_project(cas_imgs[:c], ax, v, ker=ker, out=bp_vol), record)
+        # Chunk k runs on stream k % nst with its own CAS-image buffer, so the
+        # HBM-bound image chain of one chunk overlaps the projector of the
+        # next.  Projector launches are chained with events (back projection
+        # read-modify-writes one volume; keeping the forward launches apart
+        # too keeps each launch's measured time its own).
+        main = torch.cuda.current_stream()
+        go = torch.cuda.Event()
+        go.record(main)
+        for st in streams:
+            st.wait_event(go)
+        chunks = list(range(0, p, chunk))
+        prev = None
+        for k, s0 in enumerate(chunks):
+            c = min(chunk, p - s0)
+            ax = axes[s0:s0 + c]
+            st, buf = streams[k % nst], cas_bufs[k % nst]
+            if prev is not None:
+                st.wait_event(prev)  # projector launches do not overlap each other
+            timed("fp", lambda: ops.forward_project(cas_vol, ax, m, ker=ker, out=buf[:c]), record, st)
+            prev = torch.cuda.Event()
+            prev.record(st)
+            timed("fp_chain", lambda: ops.cas_images_to_images(buf[:c], n, out=imgs[s0:s0 + c], clobber=True),
+                  record, st)
+        with torch.cuda.stream(streams[0]):
+            bp_vol.zero_()
+        for k, s0 in enumerate(chunks):
+            c = min(chunk, p - s0)
+            ax = axes[s0:s0 + c]
+            st, buf = streams[k % nst], cas_bufs[k % nst]
+            timed("bp_chain", lambda: ops.images_to_cas_images(imgs[s0:s0 + c], m, out=buf[:c]), record, st)
+            if prev is not None:
+                st.wait_event(prev)
+            timed("bp", lambda: ops.back_project(buf[:c], ax, v, ker=ker, out=bp_vol), record, st)
+            prev = torch.cuda.Event()
+            prev.record(st)
+        for st in streams:
+            done = torch.cuda.Event()
+            done.record(st)
+            main.wait_event(done)
 
     if args.calib_copy_mb:
         src = torch.ones(args.calib_copy_mb * (1 << 20) // 4, device=dev)
@@ -405,8 +438,10 @@ def main():
         "dtype": "f32",
         "data": "synthetic (seeded blob volume, create_uniform_axes grid)",
         "config": {"workload": f"plain fwd+back, N={n} interp 2 (M={m}, V={v}), {p} grid directions "
-                               f"({args.ntheta}x{args.nphi}), {chunk} directions per launch, device-resident",
-                   "N": n, "interp_factor": 2, "directions": p, "directions_per_launch": chunk},
+                               f"({args.ntheta}x{args.nphi}), {chunk} directions per launch on {nst} streams, "
+                               "device-resident",
+                   "N": n, "interp_factor": 2, "directions": p, "directions_per_launch": chunk,
+                   "streams": nst},
         "fwd_proj_per_s": round(p / ((fp_step_ms + kern["fp_chain"][1]) * 1e-3), 1),
         "back_proj_per_s": round(p / ((bp_step_ms + kern["bp_chain"][1]) * 1e-3), 1),
         "kernel_ms_per_step": {k: round(kern[k][1], 3) for k in kern},
@@ -424,7 +459,7 @@ def main():
         },
     }
     if args.serve_requests > 0:
-        del cas_imgs, imgs, bp_vol
+        del cas_bufs, imgs, bp_vol
         torch.cuda.empty_cache()
         sizes = [int(x) for x in args.serve_sizes.split(",")]
         lams = [float(x) for x in args.serve_lambdas.split(",") if x]
