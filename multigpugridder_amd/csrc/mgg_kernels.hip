@@ -320,35 +320,6 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     const float z = u2 * fi + v2 * fj + (float)cv;
     const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
 
-    // Box from the tile corners: positions are affine in (i, j), so every
-    // pixel's rounded coordinate lies in [floor(min corner), ceil(max corner)]
-    // (the 1e-3 margin covers FP32 evaluation error), widened by the stencil.
-    const float ai = (float)(i0 - ci), bi = (float)(i0 + FPT - 1 - ci);
-    const float aj = (float)(j0 - ci), bj = (float)(j0 + FPT - 1 - ci);
-    auto lo = [&](float uu, float vv) {
-        return (int)floorf(fminf(uu * ai, uu * bi) + fminf(vv * aj, vv * bj) + (float)cv - 1e-3f) - 2;
-    };
-    auto hi = [&](float uu, float vv) {
-        return (int)ceilf(fmaxf(uu * ai, uu * bi) + fmaxf(vv * aj, vv * bj) + (float)cv + 1e-3f) + 2;
-    };
-    const int bx0 = __builtin_amdgcn_readfirstlane(lo(u0, v0));
-    const int by0 = __builtin_amdgcn_readfirstlane(lo(u1, v1));
-    const int bz0 = __builtin_amdgcn_readfirstlane(lo(u2, v2));
-    const int Dx = __builtin_amdgcn_readfirstlane(hi(u0, v0)) - bx0 + 1;
-    const int Dy = __builtin_amdgcn_readfirstlane(hi(u1, v1)) - by0 + 1;
-    const int Dz = __builtin_amdgcn_readfirstlane(hi(u2, v2)) - bz0 + 1;
-    const int segs = (Dx + 3) >> 2;  // 16-byte segments per box row
-    // LDS pitches (floats), multiples of 4 (16-byte rows).  When the box
-    // still fits, the row and plane pitches are padded to an odd number of
-    // 16-byte units so that the rows and planes a tile's pixels read spread
-    // over the banks (a pitch of 16 puts every row at the same bank offset);
-    // otherwise the rows are packed.
-    int RP = (segs | 1) * 4;
-    int PP = RP * Dy + ((Dy & 1) ? 0 : RP);
-    if (PP * Dz > FPBOX) {
-        RP = segs * 4;
-        PP = RP * Dy;
-    }
     const long long V3 = (long long)V * V * V;
     const T *vol = (const T *)p.vol;
 
@@ -371,9 +342,46 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
 #pragma unroll
         for (int bb = 0; bb < 5; bb++) wij[aa][bb] = wi[aa] * wj[bb];
 
+    // Stages the voxel box of pixel rows j0 + rlo .. j0 + rhi of the tile and
+    // gathers the pixels of the lanes with mine set; false (nothing done) when
+    // the box does not fit the wave's LDS budget.  Uniform across the wave.
     float acc = 0.f;
-    if (Dx >= 5 && Dy >= 5 && Dz >= 5 && Dx <= 64 && Dy <= 64 && Dz <= 64 && PP * Dz <= FPBOX &&
-        V3 * (long long)sizeof(T) < (1LL << 32)) {
+    auto stageAndGather = [&](int rlo, int rhi, bool mine) -> bool {
+        // Box from the corners: positions are affine in (i, j), so every
+        // pixel's rounded coordinate lies in [floor(min corner), ceil(max
+        // corner)] (the 1e-3 margin covers FP32 evaluation error), widened by
+        // the stencil.
+        const float ai = (float)(i0 - ci), bi = (float)(i0 + FPT - 1 - ci);
+        const float aj = (float)(j0 + rlo - ci), bj = (float)(j0 + rhi - ci);
+        auto lo = [&](float uu, float vv) {
+            return (int)floorf(fminf(uu * ai, uu * bi) + fminf(vv * aj, vv * bj) + (float)cv - 1e-3f) - 2;
+        };
+        auto hi = [&](float uu, float vv) {
+            return (int)ceilf(fmaxf(uu * ai, uu * bi) + fmaxf(vv * aj, vv * bj) + (float)cv + 1e-3f) + 2;
+        };
+        const int bx0 = __builtin_amdgcn_readfirstlane(lo(u0, v0));
+        const int by0 = __builtin_amdgcn_readfirstlane(lo(u1, v1));
+        const int bz0 = __builtin_amdgcn_readfirstlane(lo(u2, v2));
+        const int Dx = __builtin_amdgcn_readfirstlane(hi(u0, v0)) - bx0 + 1;
+        const int Dy = __builtin_amdgcn_readfirstlane(hi(u1, v1)) - by0 + 1;
+        const int Dz = __builtin_amdgcn_readfirstlane(hi(u2, v2)) - bz0 + 1;
+        if (!(Dx >= 5 && Dy >= 5 && Dz >= 5 && Dx <= 64 && Dy <= 64 && Dz <= 64 &&
+              V3 * (long long)sizeof(T) < (1LL << 32)))
+            return false;
+        const int segs = (Dx + 3) >> 2;  // 16-byte segments per box row
+        // LDS pitches (floats), multiples of 4 (16-byte rows).  When the box
+        // still fits, the row and plane pitches are padded to an odd number
+        // of 16-byte units so that the rows and planes a tile's pixels read
+        // spread over the banks (a pitch of 16 puts every row at the same
+        // bank offset); otherwise the rows are packed.
+        int RP = (segs | 1) * 4;
+        int PP = RP * Dy + ((Dy & 1) ? 0 : RP);
+        if (PP * Dz > FPBOX) {
+            RP = segs * 4;
+            PP = RP * Dy;
+        }
+        if (PP * Dz > FPBOX) return false;
+
         // Stage: lane -> (row slot, segment); 64 / segs rows per pass.
         const int rowsPer = 64 / segs;
         const int rIn = ln / segs, sg = ln - rIn * segs;
@@ -440,7 +448,7 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
             stage(std::false_type{});
         }
         wave_lds_sync_fp();
-        if (active) {
+        if (active && mine) {
             const float *bp = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (X - 2 - bx0);
 #pragma unroll
             for (int cc = 0; cc < 5; cc++) {
@@ -454,7 +462,21 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
                 acc = fmaf(wk[cc], sc, acc);
             }
         }
-    } else if (active) {
+        wave_lds_sync_fp();  // the box is read before it is staged again
+        return true;
+    };
+
+    // The whole 8x8 tile; if its box is too large (3% of tiles for the grid
+    // directions at N=128), its two 8x4 halves one after the other; a half
+    // that still does not fit gathers from global memory.
+    bool done = stageAndGather(0, FPT - 1, true);
+    if (!done) {
+        const bool top = (ln >> 3) < FPT / 2;
+        const bool d0 = stageAndGather(0, FPT / 2 - 1, top);
+        const bool d1 = stageAndGather(FPT / 2, FPT - 1, !top);
+        done = top ? d0 : d1;
+    }
+    if (!done && active) {
 #pragma unroll
         for (int cc = 0; cc < 5; cc++) {
             float sc = 0.f;
