@@ -236,10 +236,10 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
 // ---------------------------------------------------------------------------
 constexpr int FPT = 8;          // tile edge (pixels)
 // Per-wave staged box budget (floats).  Four boxes plus the padded table take
-// 53,760 bytes of LDS, which keeps three workgroups on a CU; a larger budget
+// 51,712 bytes of LDS, which keeps three workgroups on a CU; a larger budget
 // loses the third workgroup and is slower overall.
 constexpr int FPBOX = 3072;     // per-wave staged box budget (floats): 4 boxes + table = 54,272 B, 3 workgroups per CU
-constexpr int FPKER = 1152;     // table floats kept in LDS (the padded 501-entry table fits)      // table floats kept in LDS (the padded 501-entry table fits)
+constexpr int FPKER = 640;      // table floats kept in LDS (the padded 501-entry table fits)      // table floats kept in LDS (the padded 501-entry table fits)
 
 __device__ __forceinline__ void wave_lds_sync_fp()
 {
@@ -252,7 +252,7 @@ __device__ __forceinline__ void wave_lds_sync_fp()
 // |r| <= 2.5 and the index stays within [-63, 563], inside the padding: no
 // clamp is needed.  (A non-finite coordinate gives an index the LDS bounds
 // check turns into a read of 0 or of another LDS array, never a fault.)
-constexpr int FKPAD = 320;
+constexpr int FKPAD = 64;
 __device__ __forceinline__ float kb_fast_fp(const float *__restrict__ sk, float r)
 {
     float s = r * 125.0f;
