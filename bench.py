@@ -414,6 +414,13 @@ def main():
         xs = [a.elapsed_time(b) for a, b in ev[name]]
         return float(np.mean(xs)), float(np.sum(xs)) / args.steps
 
+    def phase_ms(first, last):
+        # span from the first launch of a phase to the end of its last one, per
+        # step (the phase's launches overlap across the streams)
+        nper = len(ev[first]) // args.steps
+        spans = [ev[first][k * nper][0].elapsed_time(ev[last][(k + 1) * nper - 1][1]) for k in range(args.steps)]
+        return float(np.mean(spans))
+
     kern = {k: avg_ms(k) for k in ev}
     fp_launch_ms, fp_step_ms = kern["fp"]
     bp_launch_ms, bp_step_ms = kern["bp"]
@@ -442,8 +449,9 @@ def main():
                                "device-resident",
                    "N": n, "interp_factor": 2, "directions": p, "directions_per_launch": chunk,
                    "streams": nst},
-        "fwd_proj_per_s": round(p / ((fp_step_ms + kern["fp_chain"][1]) * 1e-3), 1),
-        "back_proj_per_s": round(p / ((bp_step_ms + kern["bp_chain"][1]) * 1e-3), 1),
+        "fwd_proj_per_s": round(p / (phase_ms("fp", "fp_chain") * 1e-3), 1),
+        "back_proj_per_s": round(p / (phase_ms("bp_chain", "bp") * 1e-3), 1),
+        # summed launch times; with several streams the chains overlap the projectors
         "kernel_ms_per_step": {k: round(kern[k][1], 3) for k in kern},
         "roofline": {
             "kernel": "back_project (bp_kernel)" if dominant == "bp" else "forward_project (fp_kernel)",
