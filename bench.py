@@ -303,7 +303,7 @@ def main():
 
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU split is described in DESIGN.md, not run)", file=sys.stderr)
+              "(more than one GPU is a design, not an implemented path)", file=sys.stderr)
         sys.exit(2)
 
     import torch
