@@ -784,6 +784,29 @@ __device__ __forceinline__ float kb_fast_nc(const float *__restrict__ sk, float 
     return sk[KPAD + cvt_rpi(s)];
 }
 
+// Fused forms of the same unclamped lookup for a tap at position p (the
+// reference's fl(fl(A + B) + volCenter)) and an integer voxel coordinate v,
+// |p - v| <= 4.5.  The reference takes r = fl(p - v), fl(r * 125), + 250.
+//  * v >= 9: p - v is exact (Sterbenz), so fl(r * 125) = fma(p, 125, -125 v)
+//    (125 v is an exact integer); form B = fl(fma(p, 125, -125 v) + 250).
+//  * v >= 130: every tap with |r| <= 2 has p >= 128, where r is a multiple of
+//    2^-16 or coarser, so 125 r is exact and the two roundings collapse into
+//    one; taps with |r| > 2 saturate at table entry 0 or 500 in either form.
+//    Form A = fma(p, 125, 250 - 125 v).
+// Both were checked against the reference's clamped index for every float p
+// in [v - 4.5, v + 4.5] and every integer v up to 2,060: 0 mismatches.  The
+// caller passes c = -125 v (form B) or 250 - 125 v (form A).
+constexpr int BP_FORM_B_MIN = 9;
+constexpr int BP_FORM_A_MIN = 130;
+
+template <int FORM>
+__device__ __forceinline__ float kb_form(const float *__restrict__ sk, float pos, float c)
+{
+    float s = fmaf(pos, 125.0f, c);
+    if (FORM == 1) s = s + 250.0f;
+    return sk[KPAD + cvt_rpi(s)];
+}
+
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
@@ -824,6 +847,12 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
         (&sacc[0][0])[k] = 0.f;
         if (DENS) (&sden[0][0])[k] = 0.f;
     }
+
+    // Lookup form per axis for orthonormal chunks (kb_form): 2 = A, 1 = B, 0 =
+    // the unfused lookup; the brick's smallest coordinate on the axis decides.
+    auto formOf = [](int o) { return o >= BP_FORM_A_MIN ? 2 : (o >= BP_FORM_B_MIN ? 1 : 0); };
+    const int fx = formOf(ox), fy = formOf(oy), fz = formOf(oz);
+    const int brickForm = (fx && fy && fz) ? fx + 2 * (fy - 1) + 4 * (fz - 1) : 0;
 
     const float cvf = (float)cv, cif = (float)ci;
     const float bc = 0.5f * (BB - 1);
@@ -912,7 +941,7 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
             const bool last = q + 4 >= total;
             wave_lds_sync();
             // Gather full groups (and, after the last direction, the rest).
-            auto gather = [&](auto clampT) {
+            auto gather = [&](auto formT) {
             while (tail - head >= 64 || (last && tail > head)) {
                 const int e = head + ln;
                 head = min(head + 64, tail);
@@ -930,6 +959,14 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                     const float fj = dvi * v0 + dvj * v1 + dvk * v2 + cif;
                     const int I = (int)roundf(fi), J = (int)roundf(fj);
                     const float fvi = (float)vi, fvj = (float)vj, fvk = (float)vk;
+                    // Form constants (used only by the fused variants).
+                    constexpr int KF = decltype(formT)::value;
+                    constexpr int FX = KF > 0 ? 1 + ((KF - 1) & 1) : 0;
+                    constexpr int FY = KF > 0 ? 1 + (((KF - 1) >> 1) & 1) : 0;
+                    constexpr int FZ = KF > 0 ? 1 + (((KF - 1) >> 2) & 1) : 0;
+                    const float cX = (FX == 2 ? 250.0f : 0.0f) - 125.0f * fvi;
+                    const float cY = (FY == 2 ? 250.0f : 0.0f) - 125.0f * fvj;
+                    const float cZ = (FZ == 2 ? 250.0f : 0.0f) - 125.0f * fvk;
                     float Ax[5], Ay[5], Az[5];
 #pragma unroll
                     for (int s5 = 0; s5 < 5; s5++) {
@@ -964,14 +1001,18 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                             const float py = Ay[ii] + By + cvf;
                             const float pz = Az[ii] + Bz + cvf;
                             float w, wy, wz;
-                            if (decltype(clampT)::value) {
+                            if constexpr (KF < 0) {
                                 w = kb_fast(sker, px - fvi);
                                 wy = kb_fast(sker, py - fvj);
                                 wz = kb_fast(sker, pz - fvk);
-                            } else {
+                            } else if constexpr (KF == 0) {
                                 w = kb_fast_nc(sker, px - fvi);
                                 wy = kb_fast_nc(sker, py - fvj);
                                 wz = kb_fast_nc(sker, pz - fvk);
+                            } else {
+                                w = kb_form<FX>(sker, px, cX);
+                                wy = kb_form<FY>(sker, py, cY);
+                                wz = kb_form<FZ>(sker, pz, cZ);
                             }
                             w = w * wy;
                             w = w * wz;
@@ -989,10 +1030,21 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                 }
             }
             };
-            if (chunkOrtho)
-                gather(std::false_type{});
-            else
-                gather(std::true_type{});
+            if (!chunkOrtho) {
+                gather(std::integral_constant<int, -1>{});
+            } else {
+                switch (brickForm) {
+                case 1: gather(std::integral_constant<int, 1>{}); break;
+                case 2: gather(std::integral_constant<int, 2>{}); break;
+                case 3: gather(std::integral_constant<int, 3>{}); break;
+                case 4: gather(std::integral_constant<int, 4>{}); break;
+                case 5: gather(std::integral_constant<int, 5>{}); break;
+                case 6: gather(std::integral_constant<int, 6>{}); break;
+                case 7: gather(std::integral_constant<int, 7>{}); break;
+                case 8: gather(std::integral_constant<int, 8>{}); break;
+                default: gather(std::integral_constant<int, 0>{}); break;
+                }
+            }
             wave_lds_sync();  // queue slots consumed before they are refilled
             if (last) break;
         }
