@@ -236,10 +236,12 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
 // ---------------------------------------------------------------------------
 constexpr int FPT = 8;          // tile edge (pixels)
 // Per-wave staged box budget (floats).  Four boxes plus the padded table take
-// 51,712 bytes of LDS, which keeps three workgroups on a CU; a larger budget
-// loses the third workgroup and is slower overall.
-constexpr int FPBOX = 3072;     // per-wave staged box budget (floats): 4 boxes + table = 54,272 B, 3 workgroups per CU
-constexpr int FPKER = 640;      // table floats kept in LDS (the padded 501-entry table fits)      // table floats kept in LDS (the padded 501-entry table fits)
+// 39,424 bytes of LDS, which keeps four workgroups on a CU (the kernel's 113
+// VGPRs allow four waves per SIMD).  A budget of 3,072 (three workgroups) was
+// 7% slower with the 16-byte gather below; 1,792 sends too many tiles to the
+// global-memory gather.
+constexpr int FPBOX = 2304;     // per-wave staged box budget (floats)
+constexpr int FPKER = 640;      // table floats kept in LDS (the padded 501-entry table fits)
 
 __device__ __forceinline__ void wave_lds_sync_fp()
 {
@@ -336,11 +338,6 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
             wk[t] = sker[kb_index((float)(Z - 2 + t) - z, p.fW, p.kerScale, p.kerCenter, p.K)];
         }
     }
-    float wij[5][5];
-#pragma unroll
-    for (int aa = 0; aa < 5; aa++)
-#pragma unroll
-        for (int bb = 0; bb < 5; bb++) wij[aa][bb] = wi[aa] * wj[bb];
 
     // Stages the voxel box of pixel rows j0 + rlo .. j0 + rhi of the tile and
     // gathers the pixels of the lanes with mine set; false (nothing done) when
@@ -447,20 +444,70 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         } else {
             stage(std::false_type{});
         }
+        // 16-byte gather: each 5-tap row is read as the two aligned 16-byte
+        // units that contain it, [xo - o, xo - o + 8) with o = xo & 3, and the
+        // x weights are shifted by o with zeros in the three unused slots.
+        // Since xo <= Dx - 5, xo - o + 8 <= 4 * ceil(Dx / 4) = 4 * segs, so
+        // the read stays inside the row's staged segments.  The unused values
+        // are staged volume values times 0, which leaves every sum
+        // bit-identical to the 5-tap read when they are finite; a non-finite
+        // one makes the pixel's sum NaN (0 * NaN and 0 * Inf are NaN).  A
+        // pixel whose sum is not finite is therefore gathered again with the
+        // 5-tap read, so NaN and Inf reach exactly the pixels whose stencil
+        // holds them, as in the reference.
         wave_lds_sync_fp();
         if (active && mine) {
-            const float *bp = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (X - 2 - bx0);
+            const int xo = X - 2 - bx0, o = xo & 3;
+            const float *bq = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (xo - o);
+            float ws[8];
+#pragma unroll
+            for (int t = 0; t < 8; t++) {
+                float w = 0.f;
+#pragma unroll
+                for (int sft = 0; sft < 4; sft++)
+                    if (t - sft >= 0 && t - sft < 5) w = (o == sft) ? wi[t - sft] : w;
+                ws[t] = w;
+            }
+            float w8[8][5];
+#pragma unroll
+            for (int t = 0; t < 8; t++)
+#pragma unroll
+                for (int bb = 0; bb < 5; bb++) w8[t][bb] = ws[t] * wj[bb];
+            float res = 0.f;
 #pragma unroll
             for (int cc = 0; cc < 5; cc++) {
                 float sc = 0.f;
 #pragma unroll
                 for (int bb = 0; bb < 5; bb++) {
-                    const float *rp = bp + cc * PP + bb * RP;
-#pragma unroll
-                    for (int aa = 0; aa < 5; aa++) sc = fmaf(wij[aa][bb], rp[aa], sc);
+                    const float *rp = bq + cc * PP + bb * RP;
+                    const f4 q0 = *(const f4 *)rp, q1 = *(const f4 *)(rp + 4);
+                    sc = fmaf(w8[0][bb], q0.x, sc);
+                    sc = fmaf(w8[1][bb], q0.y, sc);
+                    sc = fmaf(w8[2][bb], q0.z, sc);
+                    sc = fmaf(w8[3][bb], q0.w, sc);
+                    sc = fmaf(w8[4][bb], q1.x, sc);
+                    sc = fmaf(w8[5][bb], q1.y, sc);
+                    sc = fmaf(w8[6][bb], q1.z, sc);
+                    sc = fmaf(w8[7][bb], q1.w, sc);
                 }
-                acc = fmaf(wk[cc], sc, acc);
+                res = fmaf(wk[cc], sc, res);
             }
+            if (!__builtin_isfinite(res)) {
+                const float *bp = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (X - 2 - bx0);
+                res = 0.f;
+#pragma unroll
+                for (int cc = 0; cc < 5; cc++) {
+                    float sc = 0.f;
+#pragma unroll
+                    for (int bb = 0; bb < 5; bb++) {
+                        const float *rp = bp + cc * PP + bb * RP;
+#pragma unroll
+                        for (int aa = 0; aa < 5; aa++) sc = fmaf(wi[aa] * wj[bb], rp[aa], sc);
+                    }
+                    res = fmaf(wk[cc], sc, res);
+                }
+            }
+            acc = res;
         }
         wave_lds_sync_fp();  // the box is read before it is staged again
         return true;
@@ -485,7 +532,7 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
 #pragma unroll
                 for (int aa = 0; aa < 5; aa++) {
                     const long long lin = (long long)(Z - 2 + cc) * V * V + (long long)(Y - 2 + bb) * V + (X - 2 + aa);
-                    if (lin < V3 && lin >= 0) sc = fmaf(wij[aa][bb], L::one(vol + lin), sc);
+                    if (lin < V3 && lin >= 0) sc = fmaf(wi[aa] * wj[bb], L::one(vol + lin), sc);
                 }
             acc = fmaf(wk[cc], sc, acc);
         }

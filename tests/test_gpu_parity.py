@@ -56,6 +56,27 @@ def test_forward_project_f32(cuda, oracle, ker32, n, p, seed, grid):
     assert np.all(got[:, outside] == 0)
 
 
+def test_forward_project_non_finite_voxels(cuda, oracle, ker32):
+    """A NaN and an Inf voxel reach exactly the pixels whose 5x5x5 stencil
+    holds them (NaN where the Inf meets a zero table weight), as in the
+    reference; every other pixel matches the oracle."""
+    from multigpugridder_amd import ops
+
+    n, p = 32, 24
+    print(f"FP non-finite voxels n={n} p={p}", flush=True)
+    vol, cas, axes, m, v = _setup(n, p, 5, oracle)
+    c = v // 2
+    cas[c + 1, c - 2, c + 3] = np.nan
+    cas[c - 9, c + 7, c - 4] = np.inf
+    got = _np(ops.forward_project(_t(cas, cuda), _t(axes, cuda), m, ker=_t(ker32, cuda)))
+    ref = oracle.forward_project(cas, axes, ker32, m)
+    assert np.isnan(ref).any() and np.isinf(ref).any()
+    assert np.array_equal(np.isnan(got), np.isnan(ref))
+    assert np.array_equal(np.isinf(got), np.isinf(ref))
+    fin = np.isfinite(ref)
+    assert rel_l2(got[fin], ref[fin]) < TOL
+
+
 def test_forward_project_golden(cuda):
     import os
 
