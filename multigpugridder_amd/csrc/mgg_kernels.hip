@@ -235,13 +235,15 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
 // accumulation rounding differs.
 // ---------------------------------------------------------------------------
 constexpr int FPT = 8;          // tile edge (pixels)
-// Per-wave staged box budget (floats).  Four boxes plus the padded table take
-// 39,424 bytes of LDS, which keeps four workgroups on a CU (the kernel's 113
-// VGPRs allow four waves per SIMD).  A budget of 3,072 (three workgroups) was
-// 7% slower with the 16-byte gather below; 1,792 sends too many tiles to the
-// global-memory gather.
-constexpr int FPBOX = 2304;     // per-wave staged box budget (floats)
-constexpr int FPKER = 640;      // table floats kept in LDS (the padded 501-entry table fits)
+// Per-wave staged box budget (floats).  Four boxes take 40,960 bytes of LDS,
+// a quarter of the CU's 160 KB, so four workgroups share a CU (the kernel's
+// 113 VGPRs allow four waves per SIMD); 73% of the 8x8 tiles' boxes fit whole
+// on the grid and random workloads at N=128 (median 2,400 floats), the rest
+// are staged as two 8x4 halves.  Occupancy matters more than the halves'
+// extra work: three workgroups per CU with a 3,392-float budget (nearly every
+// box whole) ran 31% slower, and 2,304 floats (45% whole) 8% slower.
+constexpr int FPBOX = 2560;     // per-wave staged box budget (floats)
+constexpr int FPKER = 640;      // generic-table path: table floats kept in LDS
 
 __device__ __forceinline__ void wave_lds_sync_fp()
 {
@@ -249,19 +251,19 @@ __device__ __forceinline__ void wave_lds_sync_fp()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Padded 501-entry table lookup (see bp_fast_kernel for the exactness argument).
-// The forward gather evaluates r = (X - 2 + t) - x with X = round(x), so
-// |r| <= 2.5 and the index stays within [-63, 563], inside the padding: no
-// clamp is needed.  (A non-finite coordinate gives an index the LDS bounds
-// check turns into a read of 0 or of another LDS array, never a fault.)
-constexpr int FKPAD = 64;
-__device__ __forceinline__ float kb_fast_fp(const float *__restrict__ sk, float r)
+// 501-entry table lookup (see bp_fast_kernel for the exactness argument),
+// read from the table in global memory (2 KB, cache-resident): the forward
+// kernel does only 15 lookups per pixel, and keeping the table out of LDS
+// leaves room for a fourth workgroup per CU.  Clamping the index to [0, 500]
+// picks the entry the reference's clamp of r would (|r| <= 2.5 here), and
+// keeps any index, even from a non-finite coordinate, inside the table.
+__device__ __forceinline__ float kb_fast_fp(const float *__restrict__ ker, float r)
 {
     float s = r * 125.0f;
     s = s + 250.0f;
     int i;
     asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(s));
-    return sk[FKPAD + i];
+    return ker[min(max(i, 0), 500)];
 }
 
 template <typename L>
@@ -329,9 +331,9 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
 #pragma unroll
     for (int t = 0; t < 5; t++) {
         if (FAST) {
-            wi[t] = kb_fast_fp(sker, (float)(X - 2 + t) - x);
-            wj[t] = kb_fast_fp(sker, (float)(Y - 2 + t) - y);
-            wk[t] = kb_fast_fp(sker, (float)(Z - 2 + t) - z);
+            wi[t] = kb_fast_fp(p.ker, (float)(X - 2 + t) - x);
+            wj[t] = kb_fast_fp(p.ker, (float)(Y - 2 + t) - y);
+            wk[t] = kb_fast_fp(p.ker, (float)(Z - 2 + t) - z);
         } else {
             wi[t] = sker[kb_index((float)(X - 2 + t) - x, p.fW, p.kerScale, p.kerCenter, p.K)];
             wj[t] = sker[kb_index((float)(Y - 2 + t) - y, p.fW, p.kerScale, p.kerCenter, p.K)];
@@ -546,13 +548,10 @@ __global__ __launch_bounds__(kThreads) void fp_lds_kernel(FPArgs p)
     __shared__ float sker[FPKER];
     __shared__ __attribute__((aligned(16))) float sbox[4][FPBOX];
 
-    if (FAST) {
-        for (int t = threadIdx.x; t < 501 + 2 * FKPAD; t += kThreads)
-            sker[t] = p.ker[min(max(t - FKPAD, 0), 500)];
-    } else {
+    if (!FAST) {  // the fast lookup reads the table from global memory
         for (int t = threadIdx.x; t < p.K; t += kThreads) sker[t] = p.ker[t];
+        __syncthreads();
     }
-    __syncthreads();
     // Each wave takes work items wv + 4*blockIdx.x, + 4*gridDim.x, ...
     const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
     for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
