@@ -325,6 +325,11 @@ void chain_cas_to_images(float *casImgs, float2 *herm, float *images, int n, int
                          hipStream_t s)
 {
     if (n <= 0) return;
+    if (fused_chain_supported(m, nn)) {
+        check_launch(launch_fused_cas_to_images(casImgs, herm, images, m, nn, n, 1.0f / ((float)m * (float)m), s),
+                     "fused_cas_to_images");
+        return;
+    }
     check_launch(launch_cas_to_herm2d(casImgs, herm, m, n, s), "cas_to_herm2d");
     fft2d_batched(1, m, n, herm, sizeof(float2) * (size_t)m * (m / 2 + 1), casImgs,
                   sizeof(float) * (size_t)m * m, s);
@@ -337,6 +342,11 @@ void chain_images_to_cas(const float *images, float *casImgs, float2 *herm, int 
                          hipStream_t s, float scale, int zeroNyq)
 {
     if (n <= 0) return;
+    if (fused_chain_supported(m, nn)) {
+        check_launch(launch_fused_images_to_cas(images, herm, casImgs, m, nn, n, scale, zeroNyq, s),
+                     "fused_images_to_cas");
+        return;
+    }
     check_launch(launch_imgs_to_real2d(images, casImgs, m, nn, n, s), "imgs_to_real2d");
     fft2d_batched(0, m, n, casImgs, sizeof(float) * (size_t)m * m, herm,
                   sizeof(float2) * (size_t)m * (m / 2 + 1), s);

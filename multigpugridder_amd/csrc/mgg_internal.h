@@ -61,6 +61,12 @@ hipError_t launch_imgs_to_real2d(const float *imgs, float *real, int M, int N, i
 // zeroNyq the index-0 row and column of every CAS image are written as 0.
 hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hipStream_t s,
                                 float scale = 1.f, int zeroNyq = 0);
+// Fused two-pass image chains (M = 256): tmp holds n * N * (M/2 + 1) float2.
+bool fused_chain_supported(int M, int N);
+hipError_t launch_fused_cas_to_images(const float *cas, float2 *tmp, float *imgs, int M, int N, int n,
+                                      float scale, hipStream_t s);
+hipError_t launch_fused_images_to_cas(const float *imgs, float2 *tmp, float *cas, int M, int N, int n,
+                                      float scale, int zeroNyq, hipStream_t s);
 
 // ---- volume chains (3-D) -----------------------------------------------
 // volume (N^3) -> padded, precompensated, sign-modulated real input (M^3).

@@ -1125,89 +1125,378 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ float chk(int s) { return (s & 1) ? -1.f : 1.f; }
 
+// The 2-D chain kernels run one workgroup per (row, image): blockIdx.x is the
+// row, blockIdx.y strides over the images, the threads over the row, so every
+// index is 32-bit arithmetic on the row and column (no per-element division)
+// and each row is read and written contiguously.
+
 // CASToComplexFilter.cu:53-75 (2-D stack mode) + fftshift, half spectrum.
-__global__ void k_cas_to_herm2d(const float *__restrict__ cas, float2 *__restrict__ herm, int M,
-                                size_t total)
+__global__ void k_cas_to_herm2d(const float *__restrict__ cas, float2 *__restrict__ herm, int M, int n)
 {
     const int H = M / 2 + 1;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (size_t)gridDim.x * blockDim.x) {
-        const int cx = (int)(e % H);
-        const size_t rest = e / H;
-        const int cy = (int)(rest % M);
-        const size_t a = rest / M;
-        float2 z = make_float2(0.f, 0.f);
-        if (cx != 0 && cy != 0) {
-            const float *c = cas + a * M * M;
-            const float c1 = c[(size_t)cy * M + cx];
-            const float c2 = c[(size_t)(M - cy) * M + (M - cx)];
-            const float s = chk(cx + cy);
-            z.x = s * (0.5f * (c1 + c2));
-            z.y = s * (0.5f * (c1 - c2));
+    const int cy = blockIdx.x;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        const float *c = cas + (size_t)a * M * M;
+        float2 *h = herm + ((size_t)a * M + cy) * H;
+        for (int cx = threadIdx.x; cx < H; cx += blockDim.x) {
+            float2 z = make_float2(0.f, 0.f);
+            if (cx != 0 && cy != 0) {
+                const float c1 = c[cy * M + cx];
+                const float c2 = c[(M - cy) * M + (M - cx)];
+                const float sg = chk(cx + cy);
+                z.x = sg * (0.5f * (c1 + c2));
+                z.y = sg * (0.5f * (c1 - c2));
+            }
+            h[cx] = z;
         }
-        herm[e] = z;
     }
 }
 
 // fftshift + ComplexToReal + CropVolume + DivideScalar (gpuGridder.cpp:485-517).
 __global__ void k_real2d_to_imgs(const float *__restrict__ real, float *__restrict__ imgs, int M,
-                                 int N, float scale, size_t total)
+                                 int N, float scale, int n)
 {
     const int off = (M - N) / 2;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (size_t)gridDim.x * blockDim.x) {
-        const int px = (int)(e % N);
-        const size_t rest = e / N;
-        const int py = (int)(rest % N);
-        const size_t a = rest / N;
-        const int mx = px + off, my = py + off;
-        imgs[e] = chk(mx + my) * real[a * M * M + (size_t)my * M + mx] * scale;
+    const int py = blockIdx.x, my = py + off;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        const float *r = real + ((size_t)a * M + my) * M + off;
+        float *o = imgs + ((size_t)a * N + py) * N;
+        for (int px = threadIdx.x; px < N; px += blockDim.x)
+            o[px] = chk(px + off + my) * r[px] * scale;
     }
 }
 
 // PadVolume + RealToComplex + fftshift (gpuGridder.cpp:543-566).
 __global__ void k_imgs_to_real2d(const float *__restrict__ imgs, float *__restrict__ real, int M,
-                                 int N, size_t total)
+                                 int N, int n)
 {
     const int off = (M - N) / 2;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (size_t)gridDim.x * blockDim.x) {
-        const int mx = (int)(e % M);
-        const size_t rest = e / M;
-        const int my = (int)(rest % M);
-        const size_t a = rest / M;
-        const int px = mx - off, py = my - off;
-        float v = 0.f;
-        if (px >= 0 && px < N && py >= 0 && py < N)
-            v = chk(mx + my) * imgs[a * N * N + (size_t)py * N + px];
-        real[e] = v;
+    const int my = blockIdx.x, py = my - off;
+    const bool rowIn = py >= 0 && py < N;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        const float *im = imgs + ((size_t)a * N + (rowIn ? py : 0)) * N;
+        float *o = real + ((size_t)a * M + my) * M;
+        for (int mx = threadIdx.x; mx < M; mx += blockDim.x) {
+            const int px = mx - off;
+            float v = 0.f;
+            if (rowIn && px >= 0 && px < N) v = chk(mx + my) * im[px];
+            o[mx] = v;
+        }
     }
 }
 
 // fftshift + ComplexToCAS (gpuGridder.cpp:584-595) from the R2C half spectrum.
-__global__ void k_herm2d_to_cas(const float2 *__restrict__ herm, float *__restrict__ cas, int M,
-                                size_t total, float scale, int zeroNyq)
+__global__ void k_herm2d_to_cas(const float2 *__restrict__ herm, float *__restrict__ cas, int M, int n,
+                                float scale, int zeroNyq)
 {
     const int H = M / 2 + 1;
-    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < total;
-         e += (size_t)gridDim.x * blockDim.x) {
-        const int cx = (int)(e % M);
-        const size_t rest = e / M;
-        const int cy = (int)(rest % M);
-        const size_t a = rest / M;
-        const float2 *h = herm + a * (size_t)M * H;
-        float re, im;
-        if (cx <= M / 2) {
-            const float2 f = h[(size_t)cy * H + cx];
-            re = f.x; im = f.y;
-        } else {
-            const float2 f = h[(size_t)((M - cy) % M) * H + (M - cx)];
-            re = f.x; im = -f.y;
+    const int cy = blockIdx.x, my = (M - cy) % M;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        const float2 *h = herm + (size_t)a * M * H;
+        const float2 *hr = h + (size_t)cy * H, *hm = h + (size_t)my * H;
+        float *o = cas + ((size_t)a * M + cy) * M;
+        for (int cx = threadIdx.x; cx < M; cx += blockDim.x) {
+            float re, im;
+            if (cx <= M / 2) {
+                const float2 f = hr[cx];
+                re = f.x; im = f.y;
+            } else {
+                const float2 f = hm[M - cx];
+                re = f.x; im = -f.y;
+            }
+            const float sg = chk(cx + cy);
+            // zeroNyq: the index-0 row / column FromCAS discards (transpose of
+            // the inverse image chain); scale is exact (1) on the forward chain.
+            o[cx] = (zeroNyq && (cx == 0 || cy == 0)) ? 0.f : (sg * re + sg * im) * scale;
         }
-        const float s = chk(cx + cy);
-        // zeroNyq: the index-0 row / column FromCAS discards (transpose of the
-        // inverse image chain); scale is exact (1) on the forward chain.
-        cas[e] = (zeroNyq && (cx == 0 || cy == 0)) ? 0.f : (s * re + s * im) * scale;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Fused image chains for M = 256.  The general chains above make five or six
+// HBM passes per image (CAS -> Hermitian half spectrum, two rocFFT passes,
+// crop; or the reverse).  These make two each, with a 256-point FFT done in
+// LDS as four-step 16 x 16 (16 threads per transform, two 16-point DFTs in
+// registers and one twiddled exchange through LDS):
+//   CAS images -> images: (1) per column pair the Hermitian column is built
+//   from the CAS image as it is loaded, inverse-transformed along y, and only
+//   the N rows the crop keeps are written (N x 129 complex per image);
+//   (2) two rows at a time (a Hermitian row pair packs into one complex
+//   transform whose real and imaginary outputs are the two real rows) are
+//   inverse-transformed along x, and the kept N columns are written with the
+//   checkerboard and 1/M^2.
+//   images -> CAS images: (1) two zero-padded image rows per complex
+//   transform along x, split into the two half spectra; (2) per column the
+//   transform along y (only the N padded-in rows are nonzero), and each
+//   column's CAS value is written at (x, y) and, through the Hermitian
+//   symmetry, at the mirrored column.
+// Same arithmetic as the general chains up to FP32 rounding in the FFT
+// (twiddles from sincospif, accurate to ~1 ulp).
+// ---------------------------------------------------------------------------
+struct F16Tw { float c, s; };
+__device__ __constant__ const F16Tw kTw16[16] = {
+    {1.0f, 0.0f},
+    {0.92387953251128674f, 0.38268343236508977f},
+    {0.70710678118654752f, 0.70710678118654752f},
+    {0.38268343236508977f, 0.92387953251128674f},
+    {0.0f, 1.0f},
+    {-0.38268343236508977f, 0.92387953251128674f},
+    {-0.70710678118654752f, 0.70710678118654752f},
+    {-0.92387953251128674f, 0.38268343236508977f},
+    {-1.0f, 0.0f},
+    {-0.92387953251128674f, -0.38268343236508977f},
+    {-0.70710678118654752f, -0.70710678118654752f},
+    {-0.38268343236508977f, -0.92387953251128674f},
+    {0.0f, -1.0f},
+    {0.38268343236508977f, -0.92387953251128674f},
+    {0.70710678118654752f, -0.70710678118654752f},
+    {0.92387953251128674f, -0.38268343236508977f}};
+
+__device__ __forceinline__ float2 cmulw(float2 a, float c, float s)
+{
+    return make_float2(fmaf(a.x, c, -a.y * s), fmaf(a.x, s, a.y * c));
+}
+
+// 4-point DFT in place, X[k] = sum_n x[n] e^{S 2 pi i n k / 4}.
+template <int S>
+__device__ __forceinline__ void fft4(float2 &a, float2 &b, float2 &c, float2 &d)
+{
+    const float2 t0 = make_float2(a.x + c.x, a.y + c.y), t1 = make_float2(a.x - c.x, a.y - c.y);
+    const float2 t2 = make_float2(b.x + d.x, b.y + d.y), t3 = make_float2(b.x - d.x, b.y - d.y);
+    const float2 r = S > 0 ? make_float2(-t3.y, t3.x) : make_float2(t3.y, -t3.x);
+    a = make_float2(t0.x + t2.x, t0.y + t2.y);
+    c = make_float2(t0.x - t2.x, t0.y - t2.y);
+    b = make_float2(t1.x + r.x, t1.y + r.y);
+    d = make_float2(t1.x - r.x, t1.y - r.y);
+}
+
+// 16-point DFT in registers (radix 4 x 4).  Input v[n] in natural order;
+// output X[k2 + 4 k1] is left in v[4 k2 + k1].
+template <int S>
+__device__ __forceinline__ void fft16(float2 (&v)[16])
+{
+#pragma unroll
+    for (int n1 = 0; n1 < 4; n1++) fft4<S>(v[n1], v[n1 + 4], v[n1 + 8], v[n1 + 12]);
+#pragma unroll
+    for (int n1 = 1; n1 < 4; n1++)
+#pragma unroll
+        for (int k2 = 1; k2 < 4; k2++) {
+            const F16Tw w = kTw16[n1 * k2];
+            v[n1 + 4 * k2] = cmulw(v[n1 + 4 * k2], w.c, S > 0 ? w.s : -w.s);
+        }
+#pragma unroll
+    for (int k2 = 0; k2 < 4; k2++) fft4<S>(v[4 * k2], v[4 * k2 + 1], v[4 * k2 + 2], v[4 * k2 + 3]);
+}
+
+__device__ __forceinline__ constexpr int fft16_slot(int k) { return 4 * (k & 3) + (k >> 2); }
+
+constexpr int FFP = 17;           // exchange pitch (float2) of one 16 x 16 transform
+constexpr int FFB = 16 * FFP;     // LDS float2 per transform (272)
+
+// 256-point DFT of the 16 threads t = 0..15 of one transform.  On entry v[n2]
+// holds x[t + 16 n2]; on exit v[fft16_slot(k1)] holds X[t + 16 k1].  ex is the
+// transform's 272-float2 LDS exchange area, tw the 256 twiddles e^{2 pi i j/256}.
+// Contains workgroup barriers: every thread of the block must call it.
+template <int S>
+__device__ __forceinline__ void fft256_16(float2 (&v)[16], int t, float2 *ex, const float2 *tw)
+{
+    fft16<S>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < 16; k2++) {
+        const float2 w = tw[t * k2];
+        ex[k2 * FFP + t] = cmulw(v[fft16_slot(k2)], w.x, S > 0 ? w.y : -w.y);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int n1 = 0; n1 < 16; n1++) v[n1] = ex[t * FFP + n1];
+    fft16<S>(v);
+    __syncthreads();  // the exchange area may be reused
+}
+
+__device__ __forceinline__ void load_tw256(float2 *tw)
+{
+    float sn, cs;
+    sincospif((float)threadIdx.x / 128.0f, &sn, &cs);
+    tw[threadIdx.x] = make_float2(cs, sn);
+}
+
+// CAS images -> N kept rows of the y-inverse-transformed Hermitian columns.
+// Grid (9 column groups, images); group g covers columns 16 g .. 16 g + 15
+// (<= 128).  tmp[a][r][cx], r in [0, N), cx in [0, 129).
+__global__ __launch_bounds__(256) void k_inv_cols256(const float *__restrict__ cas, float2 *__restrict__ tmp,
+                                                     int n, int N)
+{
+    constexpr int M = 256, H = 129;
+    __shared__ float2 sx[16 * FFB];
+    __shared__ float2 tw[256];
+    load_tw256(tw);
+    const int tid = threadIdx.x, f = tid >> 4, t = tid & 15, g = blockIdx.x;
+    const int off = (M - N) / 2;
+    float2 *ex = sx + f * FFB;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        const float *c = cas + (size_t)a * M * M;
+        __syncthreads();
+        // Hermitian column values: sx[j * FFB + cy] for column 16 g + j.
+        for (int it = 0; it < 16; it++) {
+            const int idx = it * 256 + tid, cy = idx >> 4, j = idx & 15, cx = 16 * g + j;
+            float2 z = make_float2(0.f, 0.f);
+            if (cx <= M / 2 && cx != 0 && cy != 0) {
+                const float c1 = c[cy * M + cx];
+                const float c2 = c[(M - cy) * M + (M - cx)];
+                const float sg = chk(cx + cy);
+                z.x = sg * (0.5f * (c1 + c2));
+                z.y = sg * (0.5f * (c1 - c2));
+            }
+            sx[j * FFB + cy] = z;
+        }
+        __syncthreads();
+        float2 v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; n2++) v[n2] = ex[t + 16 * n2];
+        __syncthreads();
+        fft256_16<1>(v, t, ex, tw);
+        // my = t + 16 k1: park the column in natural order for the row-wise store
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * k1] = v[fft16_slot(k1)];
+        __syncthreads();
+        for (int it = 0; it < N; it += 16) {
+            const int r = it + (tid >> 4), j = tid & 15, cx = 16 * g + j;
+            if (r < N && cx <= M / 2) tmp[((size_t)a * N + r) * H + cx] = sx[j * FFB + r + off];
+        }
+    }
+}
+
+// Kept rows -> images: two rows per transform along x.  Grid (ceil(N / 32),
+// images).
+__global__ __launch_bounds__(256) void k_inv_rows256(const float2 *__restrict__ tmp, float *__restrict__ imgs,
+                                                     int n, int N, float scale)
+{
+    constexpr int M = 256, H = 129;
+    __shared__ float2 sx[16 * FFB];
+    __shared__ float2 tw[256];
+    load_tw256(tw);
+    const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
+    const int off = (M - N) / 2;
+    const int r1 = 2 * (16 * blockIdx.x + f), r2 = r1 + 1;
+    float2 *ex = sx + f * FFB;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        const float2 *g1 = tmp + ((size_t)a * N + (r1 < N ? r1 : 0)) * H;
+        const float2 *g2 = tmp + ((size_t)a * N + (r2 < N ? r2 : 0)) * H;
+        float2 v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; n2++) {
+            const int cx = t + 16 * n2;
+            float2 p1 = make_float2(0.f, 0.f), p2 = make_float2(0.f, 0.f);
+            if (cx <= M / 2) {
+                if (r1 < N) p1 = g1[cx];
+                if (r2 < N) p2 = g2[cx];
+            } else {
+                if (r1 < N) { p1 = g1[M - cx]; p1.y = -p1.y; }
+                if (r2 < N) { p2 = g2[M - cx]; p2.y = -p2.y; }
+            }
+            v[n2] = make_float2(p1.x - p2.y, p1.y + p2.x);  // row1 + i row2
+        }
+        __syncthreads();
+        fft256_16<1>(v, t, ex, tw);
+        const int my1 = r1 + off, my2 = r2 + off;
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) {
+            const int mx = t + 16 * k1, px = mx - off;
+            if (px >= 0 && px < N) {
+                const float2 z = v[fft16_slot(k1)];
+                if (r1 < N) imgs[((size_t)a * N + r1) * N + px] = chk(mx + my1) * z.x * scale;
+                if (r2 < N) imgs[((size_t)a * N + r2) * N + px] = chk(mx + my2) * z.y * scale;
+            }
+        }
+    }
+}
+
+// Images -> half spectra of the padded rows along x, two rows per transform.
+// Grid (ceil(N / 32), images); tmp[a][r][k], k in [0, 129).
+__global__ __launch_bounds__(256) void k_fwd_rows256(const float *__restrict__ imgs, float2 *__restrict__ tmp,
+                                                     int n, int N)
+{
+    constexpr int M = 256, H = 129;
+    __shared__ float2 sx[16 * FFB];
+    __shared__ float2 tw[256];
+    load_tw256(tw);
+    const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
+    const int off = (M - N) / 2;
+    const int r1 = 2 * (16 * blockIdx.x + f), r2 = r1 + 1;
+    const int my1 = r1 + off, my2 = r2 + off;
+    float2 *ex = sx + f * FFB;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        const float *i1 = imgs + ((size_t)a * N + (r1 < N ? r1 : 0)) * N;
+        const float *i2 = imgs + ((size_t)a * N + (r2 < N ? r2 : 0)) * N;
+        float2 v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; n2++) {
+            const int mx = t + 16 * n2, px = mx - off;
+            float x1 = 0.f, x2 = 0.f;
+            if (px >= 0 && px < N) {
+                if (r1 < N) x1 = chk(mx + my1) * i1[px];
+                if (r2 < N) x2 = chk(mx + my2) * i2[px];
+            }
+            v[n2] = make_float2(x1, x2);
+        }
+        __syncthreads();
+        fft256_16<-1>(v, t, ex, tw);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * k1] = v[fft16_slot(k1)];
+        __syncthreads();
+        // X1[k] = (Z[k] + conj Z[M-k]) / 2, X2[k] = (Z[k] - conj Z[M-k]) / (2i)
+        for (int k = t; k < H; k += 16) {
+            const float2 z = ex[k], zm = ex[(M - k) & (M - 1)];
+            if (r1 < N) tmp[((size_t)a * N + r1) * H + k] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+            if (r2 < N) tmp[((size_t)a * N + r2) * H + k] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
+        }
+    }
+}
+
+// Half spectra -> CAS images: transform along y per column, CAS value at the
+// column and (Hermitian) at the mirrored column.  Grid (9 column groups,
+// images).
+__global__ __launch_bounds__(256) void k_fwd_cols256(const float2 *__restrict__ tmp, float *__restrict__ cas,
+                                                     int n, int N, float scale, int zeroNyq)
+{
+    constexpr int M = 256, H = 129;
+    __shared__ float2 sx[16 * FFB];
+    __shared__ float2 tw[256];
+    load_tw256(tw);
+    const int tid = threadIdx.x, f = tid >> 4, t = tid & 15, g = blockIdx.x;
+    const int off = (M - N) / 2;
+    float2 *ex = sx + f * FFB;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        __syncthreads();
+        for (int it = 0; it < 16; it++) {
+            const int idx = it * 256 + tid, my = idx >> 4, j = idx & 15, cx = 16 * g + j, r = my - off;
+            float2 z = make_float2(0.f, 0.f);
+            if (cx <= M / 2 && r >= 0 && r < N) z = tmp[((size_t)a * N + r) * H + cx];
+            sx[j * FFB + my] = z;
+        }
+        __syncthreads();
+        float2 v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; n2++) v[n2] = ex[t + 16 * n2];
+        __syncthreads();
+        fft256_16<-1>(v, t, ex, tw);
+#pragma unroll
+        for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * k1] = v[fft16_slot(k1)];
+        __syncthreads();
+        float *o = cas + (size_t)a * M * M;
+        for (int it = 0; it < 16; it++) {
+            const int cy = it * 16 + (tid >> 4), j = tid & 15, cx = 16 * g + j;
+            if (cx > M / 2) continue;
+            const float2 fd = sx[j * FFB + cy];
+            const float sd = chk(cx + cy);
+            o[cy * M + cx] = (zeroNyq && (cx == 0 || cy == 0)) ? 0.f : (sd * fd.x + sd * fd.y) * scale;
+            if (cx != 0 && cx != M / 2) {
+                const int mx = M - cx;  // mirrored column, from F[(M - cy) % M][cx] conjugated
+                const float2 fm = sx[j * FFB + ((M - cy) & (M - 1))];
+                const float sm = chk(mx + cy);
+                o[cy * M + mx] = (zeroNyq && cy == 0) ? 0.f : (sm * fm.x - sm * fm.y) * scale;
+            }
+        }
     }
 }
 
@@ -1705,40 +1994,64 @@ hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
     return hipGetLastError();
 }
 
+// Threads per row workgroup of the 2-D chain kernels: the row length rounded
+// up to a wave, at most 256 (longer rows loop).
+static int row_threads(int len) { return std::min(256, std::max(64, (len + 63) / 64 * 64)); }
+static dim3 rows_grid(int rows, int n) { return dim3((unsigned)rows, (unsigned)std::min(n, 65535)); }
+
 hipError_t launch_cas_to_herm2d(const float *cas, float2 *herm, int M, int n, hipStream_t s)
 {
-    const size_t total = (size_t)n * M * (M / 2 + 1);
-    if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_cas_to_herm2d, dim3(grid_for(total)), dim3(kThreads), 0, s, cas, herm, M, total);
+    if (n <= 0 || M <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cas_to_herm2d, rows_grid(M, n), dim3(row_threads(M / 2 + 1)), 0, s, cas, herm, M, n);
     return hipGetLastError();
 }
 
 hipError_t launch_real2d_to_imgs(const float *real, float *imgs, int M, int N, int n, float scale,
                                  hipStream_t s)
 {
-    const size_t total = (size_t)n * N * N;
-    if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_real2d_to_imgs, dim3(grid_for(total)), dim3(kThreads), 0, s, real, imgs, M, N,
-                       scale, total);
+    if (n <= 0 || N <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_real2d_to_imgs, rows_grid(N, n), dim3(row_threads(N)), 0, s, real, imgs, M, N, scale, n);
     return hipGetLastError();
 }
 
 hipError_t launch_imgs_to_real2d(const float *imgs, float *real, int M, int N, int n, hipStream_t s)
 {
-    const size_t total = (size_t)n * M * M;
-    if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_imgs_to_real2d, dim3(grid_for(total)), dim3(kThreads), 0, s, imgs, real, M, N,
-                       total);
+    if (n <= 0 || M <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_imgs_to_real2d, rows_grid(M, n), dim3(row_threads(M)), 0, s, imgs, real, M, N, n);
     return hipGetLastError();
 }
 
 hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hipStream_t s, float scale,
                                 int zeroNyq)
 {
-    const size_t total = (size_t)n * M * M;
-    if (!total) return hipSuccess;
-    hipLaunchKernelGGL(k_herm2d_to_cas, dim3(grid_for(total)), dim3(kThreads), 0, s, herm, cas, M, total, scale,
+    if (n <= 0 || M <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_herm2d_to_cas, rows_grid(M, n), dim3(row_threads(M)), 0, s, herm, cas, M, n, scale,
                        zeroNyq);
+    return hipGetLastError();
+}
+
+bool fused_chain_supported(int M, int N) { return M == 256 && N >= 1 && N <= M; }
+
+hipError_t launch_fused_cas_to_images(const float *cas, float2 *tmp, float *imgs, int M, int N, int n,
+                                      float scale, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
+    const unsigned ny = (unsigned)std::min(n, 65535);
+    hipLaunchKernelGGL(k_inv_cols256, dim3(9, ny), dim3(256), 0, s, cas, tmp, n, N);
+    hipLaunchKernelGGL(k_inv_rows256, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, tmp, imgs, n, N,
+                       scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_fused_images_to_cas(const float *imgs, float2 *tmp, float *cas, int M, int N, int n,
+                                      float scale, int zeroNyq, hipStream_t s)
+{
+    if (n <= 0) return hipSuccess;
+    if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
+    const unsigned ny = (unsigned)std::min(n, 65535);
+    hipLaunchKernelGGL(k_fwd_rows256, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, imgs, tmp, n, N);
+    hipLaunchKernelGGL(k_fwd_cols256, dim3(9, ny), dim3(256), 0, s, tmp, cas, n, N, scale, zeroNyq);
     return hipGetLastError();
 }
 

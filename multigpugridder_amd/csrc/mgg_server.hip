@@ -346,6 +346,12 @@ struct mgg_server {
         const float scale = 1.0f / ((float)M * (float)M);
         for (int a0 = 0; a0 < D; a0 += (int)kFftBatch) {
             const int gN = std::min((int)kFftBatch, D - a0);
+            if (fused_chain_supported(M, N)) {
+                check_launch(launch_fused_cas_to_images(dCas + a0 * mm, dHerm, dImgs + a0 * nn, M, N, gN, scale,
+                                                        p.stream),
+                             "fused_cas_to_images");
+                continue;
+            }
             check_launch(launch_cas_to_herm2d(dCas + a0 * mm, dHerm, M, gN, p.stream), "cas_to_herm2d");
             fft_groups(gN, [&](int a, int cnt) { fft_exec(1, M, cnt, dHerm + a * hh, dCas + (a0 + a) * mm, p.stream); });
             check_launch(launch_real2d_to_imgs(dCas + a0 * mm, dImgs + a0 * nn, M, N, gN, scale, p.stream),
@@ -395,6 +401,12 @@ struct mgg_server {
             // forward image chain (ImgsToCASImgs), groups of kFftBatch images
             for (int a0 = 0; a0 < cnt; a0 += (int)kFftBatch) {
                 const int gN = std::min((int)kFftBatch, cnt - a0);
+                if (fused_chain_supported(M, N)) {
+                    check_launch(launch_fused_images_to_cas(imgs + a0 * nn, dHerm, dCas + a0 * mm, M, N, gN, 1.0f, 0,
+                                                            p.stream),
+                                 "fused_images_to_cas");
+                    continue;
+                }
                 check_launch(launch_imgs_to_real2d(imgs + a0 * nn, dCas + a0 * mm, M, N, gN, p.stream), "imgs_to_real2d");
                 fft_groups(gN, [&](int a, int k) { fft_exec(0, M, k, dCas + (a0 + a) * mm, dHerm + a * hh, p.stream); });
                 check_launch(launch_herm2d_to_cas(dHerm, dCas + a0 * mm, M, gN, p.stream), "herm2d_to_cas");

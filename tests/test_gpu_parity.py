@@ -271,7 +271,10 @@ def test_back_project_adjoint(cuda, oracle, ker32, n, p):
 # ---------------------------------------------------------------------------
 # Image and volume chains (rocFFT + fused pre/post kernels)
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("n,m,p", [(8, 16, 3), (16, 32, 7), (64, 128, 5), (100, 200, 3), (128, 256, 300)])
+# m = 256 runs the fused two-pass chains (any n <= m with m - n even, an odd
+# offset (m - n) / 2 included); other m run the rocFFT chains.
+@pytest.mark.parametrize("n,m,p", [(8, 16, 3), (16, 32, 7), (64, 128, 5), (100, 200, 3), (128, 256, 300),
+                                   (100, 256, 5), (6, 256, 3), (254, 256, 2), (256, 256, 2)])
 def test_image_chains(cuda, oracle, n, m, p):
     from multigpugridder_amd import ops
 
@@ -283,6 +286,20 @@ def test_image_chains(cuda, oracle, n, m, p):
     x = rng.standard_normal((p, n, n)).astype(np.float32)
     got = _np(ops.images_to_cas_images(_t(x, cuda), m))
     assert rel_l2(got, oracle.imgs_to_cas_imgs(x, m)) < TOL
+
+
+@pytest.mark.parametrize("n,m,p", [(128, 256, 40), (98, 256, 3), (24, 48, 5)])
+def test_image_chain_transposes(cuda, oracle, n, m, p):
+    """The differentiable projector's transposes of the image chains (scale and
+    index-0 zeroing), on the fused (m = 256) and the rocFFT chains."""
+    from multigpugridder_amd import ops
+
+    print(f"image chain transposes n={n} m={m} p={p}", flush=True)
+    rng = np.random.default_rng(n + 1)
+    x = rng.standard_normal((p, n, n)).astype(np.float32)
+    got = _np(ops.cas_images_to_images_transpose(_t(x, cuda), m))
+    assert rel_l2(got, oracle.cas_imgs_to_imgs_transpose(x, m)) < TOL
+    assert np.all(got[:, 0, :] == 0) and np.all(got[:, :, 0] == 0)
 
 
 @pytest.mark.parametrize("n", [8, 16, 30, 64])
