@@ -1468,6 +1468,7 @@ __global__ __launch_bounds__(256) void k_fwd_cols256(const float2 *__restrict__ 
     float2 *ex = sx + f * FFB;
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         __syncthreads();
+#pragma unroll 4  // full unrolling holds 187 VGPRs and runs slower
         for (int it = 0; it < 16; it++) {
             const int idx = it * 256 + tid, my = idx >> 4, j = idx & 15, cx = 16 * g + j, r = my - off;
             float2 z = make_float2(0.f, 0.f);
@@ -1484,6 +1485,7 @@ __global__ __launch_bounds__(256) void k_fwd_cols256(const float2 *__restrict__ 
         for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * k1] = v[fft16_slot(k1)];
         __syncthreads();
         float *o = cas + (size_t)a * M * M;
+#pragma unroll 4  // full unrolling holds 187 VGPRs and runs slower
         for (int it = 0; it < 16; it++) {
             const int cy = it * 16 + (tid >> 4), j = tid & 15, cx = 16 * g + j;
             if (cx > M / 2) continue;
