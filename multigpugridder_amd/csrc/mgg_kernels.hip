@@ -1320,55 +1320,57 @@ __device__ __forceinline__ void load_tw256(float2 *tw)
     tw[threadIdx.x] = make_float2(cs, sn);
 }
 
-// CAS images -> N kept rows of the y-inverse-transformed Hermitian columns.
-// Grid (9 column groups, images); group g covers columns 16 g .. 16 g + 15
-// (<= 128).  tmp[a][r][cx], r in [0, N), cx in [0, 129).
-__global__ __launch_bounds__(256) void k_inv_cols256(const float *__restrict__ cas, float2 *__restrict__ tmp,
+// CAS images -> R[a][ky][mx'] (ky in [0, 129), mx' in [0, N)): per spectrum row
+// ky the full complex row is built from CAS rows ky and M - ky as they are
+// read (whole rows), transformed along x, and the N kept columns are stored.
+// Rows ky > M/2 are not needed: R[M - ky] = conj R[ky].  Grid (9 row groups of
+// 16, images).
+__global__ __launch_bounds__(256) void k_inv_rows256(const float *__restrict__ cas, float2 *__restrict__ tmp,
                                                      int n, int N)
 {
     constexpr int M = 256, H = 129;
     __shared__ float2 sx[16 * FFB];
     __shared__ float2 tw[256];
     load_tw256(tw);
-    const int tid = threadIdx.x, f = tid >> 4, t = tid & 15, g = blockIdx.x;
+    const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
     const int off = (M - N) / 2;
+    const int ky = 16 * blockIdx.x + f;
+    const bool rowOk = ky <= M / 2;
     float2 *ex = sx + f * FFB;
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
-        const float *c = cas + (size_t)a * M * M;
-        __syncthreads();
-        // Hermitian column values: sx[j * FFB + cy] for column 16 g + j.
-        for (int it = 0; it < 16; it++) {
-            const int idx = it * 256 + tid, cy = idx >> 4, j = idx & 15, cx = 16 * g + j;
+        const float *rowA = cas + (size_t)a * M * M + (size_t)(rowOk ? ky : 0) * M;
+        const float *rowB = cas + (size_t)a * M * M + (size_t)((M - ky) & (M - 1)) * M;
+        float2 v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; n2++) {
+            const int kx = t + 16 * n2;
             float2 z = make_float2(0.f, 0.f);
-            if (cx <= M / 2 && cx != 0 && cy != 0) {
-                const float c1 = c[cy * M + cx];
-                const float c2 = c[(M - cy) * M + (M - cx)];
-                const float sg = chk(cx + cy);
+            if (rowOk && kx != 0 && ky != 0) {
+                const float c1 = rowA[kx], c2 = rowB[M - kx];
+                const float sg = chk(kx + ky);
                 z.x = sg * (0.5f * (c1 + c2));
                 z.y = sg * (0.5f * (c1 - c2));
             }
-            sx[j * FFB + cy] = z;
+            v[n2] = z;
         }
         __syncthreads();
-        float2 v[16];
-#pragma unroll
-        for (int n2 = 0; n2 < 16; n2++) v[n2] = ex[t + 16 * n2];
-        __syncthreads();
         fft256_16<1>(v, t, ex, tw);
-        // my = t + 16 k1: park the column in natural order for the row-wise store
+        if (rowOk) {
+            float2 *o = tmp + ((size_t)a * H + ky) * N;
 #pragma unroll
-        for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * k1] = v[fft16_slot(k1)];
-        __syncthreads();
-        for (int it = 0; it < N; it += 16) {
-            const int r = it + (tid >> 4), j = tid & 15, cx = 16 * g + j;
-            if (r < N && cx <= M / 2) tmp[((size_t)a * N + r) * H + cx] = sx[j * FFB + r + off];
+            for (int k1 = 0; k1 < 16; k1++) {
+                const int px = t + 16 * k1 - off;
+                if (px >= 0 && px < N) o[px] = v[fft16_slot(k1)];
+            }
         }
     }
 }
 
-// Kept rows -> images: two rows per transform along x.  Grid (ceil(N / 32),
+// R -> images: 32 columns per workgroup, two Hermitian columns packed into one
+// complex transform along y (its real and imaginary outputs are the two real
+// columns); output through LDS as whole row segments.  Grid (ceil(N / 32),
 // images).
-__global__ __launch_bounds__(256) void k_inv_rows256(const float2 *__restrict__ tmp, float *__restrict__ imgs,
+__global__ __launch_bounds__(256) void k_inv_cols256(const float2 *__restrict__ tmp, float *__restrict__ imgs,
                                                      int n, int N, float scale)
 {
     constexpr int M = 256, H = 129;
@@ -1377,43 +1379,53 @@ __global__ __launch_bounds__(256) void k_inv_rows256(const float2 *__restrict__ 
     load_tw256(tw);
     const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
     const int off = (M - N) / 2;
-    const int r1 = 2 * (16 * blockIdx.x + f), r2 = r1 + 1;
+    const int c0 = 32 * blockIdx.x;
     float2 *ex = sx + f * FFB;
+    float *so = reinterpret_cast<float *>(sx);  // output staging, pitch 33 floats
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
-        const float2 *g1 = tmp + ((size_t)a * N + (r1 < N ? r1 : 0)) * H;
-        const float2 *g2 = tmp + ((size_t)a * N + (r2 < N ? r2 : 0)) * H;
-        float2 v[16];
-#pragma unroll
-        for (int n2 = 0; n2 < 16; n2++) {
-            const int cx = t + 16 * n2;
-            float2 p1 = make_float2(0.f, 0.f), p2 = make_float2(0.f, 0.f);
-            if (cx <= M / 2) {
-                if (r1 < N) p1 = g1[cx];
-                if (r2 < N) p2 = g2[cx];
-            } else {
-                if (r1 < N) { p1 = g1[M - cx]; p1.y = -p1.y; }
-                if (r2 < N) { p2 = g2[M - cx]; p2.y = -p2.y; }
+        __syncthreads();
+        // Z_f[ky] = R[ky][c0 + 2f] + i R[ky][c0 + 2f + 1], and by symmetry
+        // Z_f[M - ky] = conj R[ky][2f] + i conj R[ky][2f + 1].
+        for (int idx = tid; idx < H * 32; idx += 256) {
+            const int ky = idx >> 5, col = idx & 31, px = c0 + col;
+            float2 val = make_float2(0.f, 0.f);
+            if (px < N) val = tmp[((size_t)a * H + ky) * N + px];
+            const float ox = __shfl_xor(val.x, 1), oy = __shfl_xor(val.y, 1);
+            if (!(col & 1)) {
+                float2 *zf = sx + (col >> 1) * FFB;
+                zf[ky] = make_float2(val.x - oy, val.y + ox);
+                if (ky != 0 && ky != M / 2) zf[M - ky] = make_float2(val.x + oy, ox - val.y);
             }
-            v[n2] = make_float2(p1.x - p2.y, p1.y + p2.x);  // row1 + i row2
         }
         __syncthreads();
+        float2 v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; n2++) v[n2] = ex[t + 16 * n2];
+        __syncthreads();
         fft256_16<1>(v, t, ex, tw);
-        const int my1 = r1 + off, my2 = r2 + off;
 #pragma unroll
         for (int k1 = 0; k1 < 16; k1++) {
-            const int mx = t + 16 * k1, px = mx - off;
-            if (px >= 0 && px < N) {
+            const int my = t + 16 * k1, py = my - off;
+            if (py >= 0 && py < N) {
                 const float2 z = v[fft16_slot(k1)];
-                if (r1 < N) imgs[((size_t)a * N + r1) * N + px] = chk(mx + my1) * z.x * scale;
-                if (r2 < N) imgs[((size_t)a * N + r2) * N + px] = chk(mx + my2) * z.y * scale;
+                const int mx = c0 + 2 * f + off;
+                so[py * 33 + 2 * f] = chk(mx + my) * z.x * scale;
+                so[py * 33 + 2 * f + 1] = chk(mx + 1 + my) * z.y * scale;
             }
+        }
+        __syncthreads();
+        for (int idx = tid; idx < N * 32; idx += 256) {
+            const int py = idx >> 5, col = idx & 31, px = c0 + col;
+            if (px < N) imgs[((size_t)a * N + py) * N + px] = so[py * 33 + col];
         }
     }
 }
 
-// Images -> half spectra of the padded rows along x, two rows per transform.
-// Grid (ceil(N / 32), images); tmp[a][r][k], k in [0, 129).
-__global__ __launch_bounds__(256) void k_fwd_rows256(const float *__restrict__ imgs, float2 *__restrict__ tmp,
+// Images -> C[a][ky][px] (ky in [0, 129)): 32 image columns per workgroup, two
+// zero-padded real columns per complex transform along y, split into the two
+// half spectra; input through LDS from whole row segments.  Grid
+// (ceil(N / 32), images).
+__global__ __launch_bounds__(256) void k_fwd_cols256(const float *__restrict__ imgs, float2 *__restrict__ tmp,
                                                      int n, int N)
 {
     constexpr int M = 256, H = 129;
@@ -1422,58 +1434,17 @@ __global__ __launch_bounds__(256) void k_fwd_rows256(const float *__restrict__ i
     load_tw256(tw);
     const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
     const int off = (M - N) / 2;
-    const int r1 = 2 * (16 * blockIdx.x + f), r2 = r1 + 1;
-    const int my1 = r1 + off, my2 = r2 + off;
+    const int c0 = 32 * blockIdx.x;
     float2 *ex = sx + f * FFB;
-    for (int a = blockIdx.y; a < n; a += gridDim.y) {
-        const float *i1 = imgs + ((size_t)a * N + (r1 < N ? r1 : 0)) * N;
-        const float *i2 = imgs + ((size_t)a * N + (r2 < N ? r2 : 0)) * N;
-        float2 v[16];
-#pragma unroll
-        for (int n2 = 0; n2 < 16; n2++) {
-            const int mx = t + 16 * n2, px = mx - off;
-            float x1 = 0.f, x2 = 0.f;
-            if (px >= 0 && px < N) {
-                if (r1 < N) x1 = chk(mx + my1) * i1[px];
-                if (r2 < N) x2 = chk(mx + my2) * i2[px];
-            }
-            v[n2] = make_float2(x1, x2);
-        }
-        __syncthreads();
-        fft256_16<-1>(v, t, ex, tw);
-#pragma unroll
-        for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * k1] = v[fft16_slot(k1)];
-        __syncthreads();
-        // X1[k] = (Z[k] + conj Z[M-k]) / 2, X2[k] = (Z[k] - conj Z[M-k]) / (2i)
-        for (int k = t; k < H; k += 16) {
-            const float2 z = ex[k], zm = ex[(M - k) & (M - 1)];
-            if (r1 < N) tmp[((size_t)a * N + r1) * H + k] = make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
-            if (r2 < N) tmp[((size_t)a * N + r2) * H + k] = make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x));
-        }
-    }
-}
-
-// Half spectra -> CAS images: transform along y per column, CAS value at the
-// column and (Hermitian) at the mirrored column.  Grid (9 column groups,
-// images).
-__global__ __launch_bounds__(256) void k_fwd_cols256(const float2 *__restrict__ tmp, float *__restrict__ cas,
-                                                     int n, int N, float scale, int zeroNyq)
-{
-    constexpr int M = 256, H = 129;
-    __shared__ float2 sx[16 * FFB];
-    __shared__ float2 tw[256];
-    load_tw256(tw);
-    const int tid = threadIdx.x, f = tid >> 4, t = tid & 15, g = blockIdx.x;
-    const int off = (M - N) / 2;
-    float2 *ex = sx + f * FFB;
+    float *sf = reinterpret_cast<float *>(sx);
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         __syncthreads();
-#pragma unroll 4  // full unrolling holds 187 VGPRs and runs slower
-        for (int it = 0; it < 16; it++) {
-            const int idx = it * 256 + tid, my = idx >> 4, j = idx & 15, cx = 16 * g + j, r = my - off;
-            float2 z = make_float2(0.f, 0.f);
-            if (cx <= M / 2 && r >= 0 && r < N) z = tmp[((size_t)a * N + r) * H + cx];
-            sx[j * FFB + my] = z;
+        // column c0 + col goes to transform col / 2, real (even) or imaginary part
+        for (int idx = tid; idx < M * 32; idx += 256) {
+            const int my = idx >> 5, col = idx & 31, px = c0 + col, py = my - off;
+            float x = 0.f;
+            if (px < N && py >= 0 && py < N) x = chk(px + off + my) * imgs[((size_t)a * N + py) * N + px];
+            sf[((col >> 1) * FFB + my) * 2 + (col & 1)] = x;
         }
         __syncthreads();
         float2 v[16];
@@ -1484,19 +1455,58 @@ __global__ __launch_bounds__(256) void k_fwd_cols256(const float2 *__restrict__ 
 #pragma unroll
         for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * k1] = v[fft16_slot(k1)];
         __syncthreads();
-        float *o = cas + (size_t)a * M * M;
-#pragma unroll 4  // full unrolling holds 187 VGPRs and runs slower
-        for (int it = 0; it < 16; it++) {
-            const int cy = it * 16 + (tid >> 4), j = tid & 15, cx = 16 * g + j;
-            if (cx > M / 2) continue;
-            const float2 fd = sx[j * FFB + cy];
-            const float sd = chk(cx + cy);
-            o[cy * M + cx] = (zeroNyq && (cx == 0 || cy == 0)) ? 0.f : (sd * fd.x + sd * fd.y) * scale;
-            if (cx != 0 && cx != M / 2) {
-                const int mx = M - cx;  // mirrored column, from F[(M - cy) % M][cx] conjugated
-                const float2 fm = sx[j * FFB + ((M - cy) & (M - 1))];
-                const float sm = chk(mx + cy);
-                o[cy * M + mx] = (zeroNyq && cy == 0) ? 0.f : (sm * fm.x - sm * fm.y) * scale;
+        // C_even[ky] = (Z[ky] + conj Z[M-ky]) / 2, C_odd[ky] = (Z[ky] - conj Z[M-ky]) / (2i)
+        for (int idx = tid; idx < H * 32; idx += 256) {
+            const int ky = idx >> 5, col = idx & 31, px = c0 + col;
+            if (px >= N) continue;
+            const float2 *zf = sx + (col >> 1) * FFB;
+            const float2 z = zf[ky], zm = zf[(M - ky) & (M - 1)];
+            tmp[((size_t)a * H + ky) * N + px] = (col & 1)
+                ? make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x))
+                : make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
+        }
+    }
+}
+
+// C -> CAS images: per spectrum row ky in [0, 129) the transform along x gives
+// the full row F[ky][.]; CAS row ky and (Hermitian) row M - ky are written
+// whole.  Grid (9 row groups of 16, images).
+__global__ __launch_bounds__(256) void k_fwd_rows256(const float2 *__restrict__ tmp, float *__restrict__ cas,
+                                                     int n, int N, float scale, int zeroNyq)
+{
+    constexpr int M = 256, H = 129;
+    __shared__ float2 sx[16 * FFB];
+    __shared__ float2 tw[256];
+    load_tw256(tw);
+    const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
+    const int off = (M - N) / 2;
+    const int ky = 16 * blockIdx.x + f;
+    const bool rowOk = ky <= M / 2;
+    float2 *ex = sx + f * FFB;
+    for (int a = blockIdx.y; a < n; a += gridDim.y) {
+        const float2 *crow = tmp + ((size_t)a * H + (rowOk ? ky : 0)) * N;
+        float2 v[16];
+#pragma unroll
+        for (int n2 = 0; n2 < 16; n2++) {
+            const int px = t + 16 * n2 - off;
+            v[n2] = (rowOk && px >= 0 && px < N) ? crow[px] : make_float2(0.f, 0.f);
+        }
+        __syncthreads();
+        fft256_16<-1>(v, t, ex, tw);
+        if (rowOk) {
+            float *o = cas + (size_t)a * M * M;
+            const bool mirror = ky != 0 && ky != M / 2;
+#pragma unroll
+            for (int k1 = 0; k1 < 16; k1++) {
+                const int kx = t + 16 * k1;
+                const float2 F = v[fft16_slot(k1)];
+                const float sd = chk(kx + ky);
+                o[ky * M + kx] = (zeroNyq && (kx == 0 || ky == 0)) ? 0.f : (sd * F.x + sd * F.y) * scale;
+                if (mirror) {
+                    const int x = (M - kx) & (M - 1);  // CAS[M - ky][x] from conj F[ky][(M - x) % M]
+                    const float sm = chk(x + ky);
+                    o[(M - ky) * M + x] = (zeroNyq && x == 0) ? 0.f : (sm * F.x - sm * F.y) * scale;
+                }
             }
         }
     }
@@ -2040,8 +2050,8 @@ hipError_t launch_fused_cas_to_images(const float *cas, float2 *tmp, float *imgs
     if (n <= 0) return hipSuccess;
     if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
     const unsigned ny = (unsigned)std::min(n, 65535);
-    hipLaunchKernelGGL(k_inv_cols256, dim3(9, ny), dim3(256), 0, s, cas, tmp, n, N);
-    hipLaunchKernelGGL(k_inv_rows256, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, tmp, imgs, n, N,
+    hipLaunchKernelGGL(k_inv_rows256, dim3(9, ny), dim3(256), 0, s, cas, tmp, n, N);
+    hipLaunchKernelGGL(k_inv_cols256, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, tmp, imgs, n, N,
                        scale);
     return hipGetLastError();
 }
@@ -2052,8 +2062,8 @@ hipError_t launch_fused_images_to_cas(const float *imgs, float2 *tmp, float *cas
     if (n <= 0) return hipSuccess;
     if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
     const unsigned ny = (unsigned)std::min(n, 65535);
-    hipLaunchKernelGGL(k_fwd_rows256, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, imgs, tmp, n, N);
-    hipLaunchKernelGGL(k_fwd_cols256, dim3(9, ny), dim3(256), 0, s, tmp, cas, n, N, scale, zeroNyq);
+    hipLaunchKernelGGL(k_fwd_cols256, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, imgs, tmp, n, N);
+    hipLaunchKernelGGL(k_fwd_rows256, dim3(9, ny), dim3(256), 0, s, tmp, cas, n, N, scale, zeroNyq);
     return hipGetLastError();
 }
 
