@@ -1213,23 +1213,24 @@ __global__ void k_herm2d_to_cas(const float2 *__restrict__ herm, float *__restri
 }
 
 // ---------------------------------------------------------------------------
-// Fused image chains for M = 256.  The general chains above make five or six
-// HBM passes per image (CAS -> Hermitian half spectrum, two rocFFT passes,
-// crop; or the reverse).  These make two each, with a 256-point FFT done in
-// LDS as four-step 16 x 16 (16 threads per transform, two 16-point DFTs in
-// registers and one twiddled exchange through LDS):
-//   CAS images -> images: (1) per column pair the Hermitian column is built
-//   from the CAS image as it is loaded, inverse-transformed along y, and only
-//   the N rows the crop keeps are written (N x 129 complex per image);
-//   (2) two rows at a time (a Hermitian row pair packs into one complex
-//   transform whose real and imaginary outputs are the two real rows) are
-//   inverse-transformed along x, and the kept N columns are written with the
-//   checkerboard and 1/M^2.
-//   images -> CAS images: (1) two zero-padded image rows per complex
-//   transform along x, split into the two half spectra; (2) per column the
-//   transform along y (only the N padded-in rows are nonzero), and each
-//   column's CAS value is written at (x, y) and, through the Hermitian
-//   symmetry, at the mirrored column.
+// Fused image chains for M = 256 and 512.  The general chains above make five
+// or six HBM passes per image (CAS -> Hermitian half spectrum, two rocFFT
+// passes, crop; or the reverse).  These make two each, with the M-point FFT
+// done in LDS as four-step 16 x R (M = 16 R; 16 threads per transform, the
+// R-point and 16-point DFTs in registers, one twiddled exchange through LDS;
+// the 32-point DFT is two 16-point DFTs and a radix-2 step), and are ordered
+// so that CAS images are read and written as whole rows:
+//   CAS images -> images: (1) k_inv_rows: per spectrum row ky <= M/2 the full
+//   complex row is built from CAS rows ky and M - ky as they are read,
+//   inverse-transformed along x, and the N cropped columns are kept
+//   ((M/2 + 1) x N complex per image; rows ky > M/2 follow by Hermitian
+//   symmetry); (2) k_inv_cols: 32 columns per workgroup, two Hermitian
+//   columns per complex transform along y (its real and imaginary outputs are
+//   the two real columns), checkerboard and 1/M^2, written through LDS.
+//   images -> CAS images: (1) k_fwd_cols: two zero-padded image columns per
+//   complex transform along y, split into two half spectra; (2) k_fwd_rows:
+//   per spectrum row ky <= M/2 the transform along x gives the full row, and
+//   CAS rows ky and M - ky are written whole.
 // Same arithmetic as the general chains up to FP32 rounding in the FFT
 // (twiddles from sincospif, accurate to ~1 ulp).
 // ---------------------------------------------------------------------------
@@ -1290,59 +1291,132 @@ __device__ __forceinline__ void fft16(float2 (&v)[16])
 
 __device__ __forceinline__ constexpr int fft16_slot(int k) { return 4 * (k & 3) + (k >> 2); }
 
-constexpr int FFP = 17;           // exchange pitch (float2) of one 16 x 16 transform
-constexpr int FFB = 16 * FFP;     // LDS float2 per transform (272)
+// cos, sin of 2 pi j / 32, j = 0..15
+__device__ __constant__ const F16Tw kTw32[16] = {
+    {1.0f, 0.0f},
+    {0.98078528040323043f, 0.19509032201612825f},
+    {0.92387953251128674f, 0.38268343236508977f},
+    {0.83146961230254524f, 0.55557023301960218f},
+    {0.70710678118654752f, 0.70710678118654752f},
+    {0.55557023301960218f, 0.83146961230254524f},
+    {0.38268343236508977f, 0.92387953251128674f},
+    {0.19509032201612825f, 0.98078528040323043f},
+    {0.0f, 1.0f},
+    {-0.19509032201612825f, 0.98078528040323043f},
+    {-0.38268343236508977f, 0.92387953251128674f},
+    {-0.55557023301960218f, 0.83146961230254524f},
+    {-0.70710678118654752f, 0.70710678118654752f},
+    {-0.83146961230254524f, 0.55557023301960218f},
+    {-0.92387953251128674f, 0.38268343236508977f},
+    {-0.98078528040323043f, 0.19509032201612825f}};
 
-// 256-point DFT of the 16 threads t = 0..15 of one transform.  On entry v[n2]
-// holds x[t + 16 n2]; on exit v[fft16_slot(k1)] holds X[t + 16 k1].  ex is the
-// transform's 272-float2 LDS exchange area, tw the 256 twiddles e^{2 pi i j/256}.
-// Contains workgroup barriers: every thread of the block must call it.
+// 32-point DFT in registers: two 16-point DFTs (even and odd inputs) and a
+// radix-2 combination.  X[k] is left in v[fft16_slot(k)], X[k + 16] in
+// v[16 + fft16_slot(k)].
 template <int S>
-__device__ __forceinline__ void fft256_16(float2 (&v)[16], int t, float2 *ex, const float2 *tw)
+__device__ __forceinline__ void fft32(float2 (&v)[32])
 {
-    fft16<S>(v);
+    float2 e[16], o[16];
 #pragma unroll
-    for (int k2 = 0; k2 < 16; k2++) {
+    for (int i = 0; i < 16; i++) {
+        e[i] = v[2 * i];
+        o[i] = v[2 * i + 1];
+    }
+    fft16<S>(e);
+    fft16<S>(o);
+#pragma unroll
+    for (int k = 0; k < 16; k++) {
+        const F16Tw w = kTw32[k];
+        const float2 E = e[fft16_slot(k)], O = cmulw(o[fft16_slot(k)], w.c, S > 0 ? w.s : -w.s);
+        v[fft16_slot(k)] = make_float2(E.x + O.x, E.y + O.y);
+        v[16 + fft16_slot(k)] = make_float2(E.x - O.x, E.y - O.y);
+    }
+}
+
+template <int R, int S>
+__device__ __forceinline__ void fft_regs(float2 (&v)[R])
+{
+    if constexpr (R == 16) fft16<S>(v);
+    else fft32<S>(v);
+}
+
+// Position of X[k] in v[] after fft_regs<R>.
+template <int R>
+__device__ __forceinline__ constexpr int fftR_slot(int k)
+{
+    return R == 16 ? fft16_slot(k) : (k < 16 ? fft16_slot(k) : 16 + fft16_slot(k - 16));
+}
+
+constexpr int FFP = 17;  // exchange pitch (float2)
+template <int R> struct FftGeom {
+    static constexpr int M = 16 * R, H = M / 2 + 1;
+    static constexpr int FFB = R * FFP;  // LDS float2 per transform (>= M)
+};
+
+// M-point DFT (M = 16 R) by the 16 threads t = 0..15 of one transform,
+// four-step 16 x R.  On entry v[n2] holds x[t + 16 n2] (n2 < R); on exit
+// v[16 j + fft16_slot(k1)] holds X[t + 16 j + R k1] (j < R / 16, k1 < 16).
+// ex is the transform's FFB-float2 LDS exchange area, tw the M twiddles
+// e^{2 pi i q / M}.  Contains workgroup barriers: every thread of the block
+// must call it.
+template <int R, int S>
+__device__ __forceinline__ void fft_m(float2 (&v)[R], int t, float2 *ex, const float2 *tw)
+{
+    fft_regs<R, S>(v);
+#pragma unroll
+    for (int k2 = 0; k2 < R; k2++) {
         const float2 w = tw[t * k2];
-        ex[k2 * FFP + t] = cmulw(v[fft16_slot(k2)], w.x, S > 0 ? w.y : -w.y);
+        ex[k2 * FFP + t] = cmulw(v[fftR_slot<R>(k2)], w.x, S > 0 ? w.y : -w.y);
     }
     __syncthreads();
 #pragma unroll
-    for (int n1 = 0; n1 < 16; n1++) v[n1] = ex[t * FFP + n1];
-    fft16<S>(v);
+    for (int j = 0; j < R / 16; j++) {
+        float2 u[16];
+#pragma unroll
+        for (int n1 = 0; n1 < 16; n1++) u[n1] = ex[(t + 16 * j) * FFP + n1];
+        fft16<S>(u);
+#pragma unroll
+        for (int q = 0; q < 16; q++) v[16 * j + q] = u[q];
+    }
     __syncthreads();  // the exchange area may be reused
 }
 
-__device__ __forceinline__ void load_tw256(float2 *tw)
+template <int R>
+__device__ __forceinline__ void load_tw(float2 *tw)
 {
-    float sn, cs;
-    sincospif((float)threadIdx.x / 128.0f, &sn, &cs);
-    tw[threadIdx.x] = make_float2(cs, sn);
+    constexpr int M = 16 * R;
+    for (int q = threadIdx.x; q < M; q += blockDim.x) {
+        float sn, cs;
+        sincospif((float)q / (float)(M / 2), &sn, &cs);
+        tw[q] = make_float2(cs, sn);
+    }
 }
 
-// CAS images -> R[a][ky][mx'] (ky in [0, 129), mx' in [0, N)): per spectrum row
-// ky the full complex row is built from CAS rows ky and M - ky as they are
+// CAS images -> R[a][ky][mx'] (ky in [0, M/2 + 1), mx' in [0, N)): per spectrum
+// row ky the full complex row is built from CAS rows ky and M - ky as they are
 // read (whole rows), transformed along x, and the N kept columns are stored.
-// Rows ky > M/2 are not needed: R[M - ky] = conj R[ky].  Grid (9 row groups of
+// Rows ky > M/2 are not needed: R[M - ky] = conj R[ky].  Grid (row groups of
 // 16, images).
-__global__ __launch_bounds__(256) void k_inv_rows256(const float *__restrict__ cas, float2 *__restrict__ tmp,
-                                                     int n, int N)
+template <int R>
+__global__ __launch_bounds__(256) void k_inv_rows(const float *__restrict__ cas, float2 *__restrict__ tmp,
+                                                  int n, int N)
 {
-    constexpr int M = 256, H = 129;
-    __shared__ float2 sx[16 * FFB];
-    __shared__ float2 tw[256];
-    load_tw256(tw);
+    using G = FftGeom<R>;
+    constexpr int M = G::M, H = G::H;
+    __shared__ float2 sx[16 * G::FFB];
+    __shared__ float2 tw[M];
+    load_tw<R>(tw);
     const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
     const int off = (M - N) / 2;
     const int ky = 16 * blockIdx.x + f;
     const bool rowOk = ky <= M / 2;
-    float2 *ex = sx + f * FFB;
+    float2 *ex = sx + f * G::FFB;
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         const float *rowA = cas + (size_t)a * M * M + (size_t)(rowOk ? ky : 0) * M;
         const float *rowB = cas + (size_t)a * M * M + (size_t)((M - ky) & (M - 1)) * M;
-        float2 v[16];
+        float2 v[R];
 #pragma unroll
-        for (int n2 = 0; n2 < 16; n2++) {
+        for (int n2 = 0; n2 < R; n2++) {
             const int kx = t + 16 * n2;
             float2 z = make_float2(0.f, 0.f);
             if (rowOk && kx != 0 && ky != 0) {
@@ -1354,14 +1428,16 @@ __global__ __launch_bounds__(256) void k_inv_rows256(const float *__restrict__ c
             v[n2] = z;
         }
         __syncthreads();
-        fft256_16<1>(v, t, ex, tw);
+        fft_m<R, 1>(v, t, ex, tw);
         if (rowOk) {
             float2 *o = tmp + ((size_t)a * H + ky) * N;
 #pragma unroll
-            for (int k1 = 0; k1 < 16; k1++) {
-                const int px = t + 16 * k1 - off;
-                if (px >= 0 && px < N) o[px] = v[fft16_slot(k1)];
-            }
+            for (int j = 0; j < R / 16; j++)
+#pragma unroll
+                for (int k1 = 0; k1 < 16; k1++) {
+                    const int px = t + 16 * j + R * k1 - off;
+                    if (px >= 0 && px < N) o[px] = v[16 * j + fft16_slot(k1)];
+                }
         }
     }
 }
@@ -1370,18 +1446,20 @@ __global__ __launch_bounds__(256) void k_inv_rows256(const float *__restrict__ c
 // complex transform along y (its real and imaginary outputs are the two real
 // columns); output through LDS as whole row segments.  Grid (ceil(N / 32),
 // images).
-__global__ __launch_bounds__(256) void k_inv_cols256(const float2 *__restrict__ tmp, float *__restrict__ imgs,
-                                                     int n, int N, float scale)
+template <int R>
+__global__ __launch_bounds__(256) void k_inv_cols(const float2 *__restrict__ tmp, float *__restrict__ imgs,
+                                                  int n, int N, float scale)
 {
-    constexpr int M = 256, H = 129;
-    __shared__ float2 sx[16 * FFB];
-    __shared__ float2 tw[256];
-    load_tw256(tw);
+    using G = FftGeom<R>;
+    constexpr int M = G::M, H = G::H;
+    __shared__ float2 sx[16 * G::FFB];
+    __shared__ float2 tw[M];
+    load_tw<R>(tw);
     const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
     const int off = (M - N) / 2;
     const int c0 = 32 * blockIdx.x;
-    float2 *ex = sx + f * FFB;
-    float *so = reinterpret_cast<float *>(sx);  // output staging, pitch 33 floats
+    float2 *ex = sx + f * G::FFB;
+    float *so = reinterpret_cast<float *>(sx);  // output staging, pitch 33 floats (N * 33 <= 32 * FFB)
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         __syncthreads();
         // Z_f[ky] = R[ky][c0 + 2f] + i R[ky][c0 + 2f + 1], and by symmetry
@@ -1392,27 +1470,29 @@ __global__ __launch_bounds__(256) void k_inv_cols256(const float2 *__restrict__ 
             if (px < N) val = tmp[((size_t)a * H + ky) * N + px];
             const float ox = __shfl_xor(val.x, 1), oy = __shfl_xor(val.y, 1);
             if (!(col & 1)) {
-                float2 *zf = sx + (col >> 1) * FFB;
+                float2 *zf = sx + (col >> 1) * G::FFB;
                 zf[ky] = make_float2(val.x - oy, val.y + ox);
                 if (ky != 0 && ky != M / 2) zf[M - ky] = make_float2(val.x + oy, ox - val.y);
             }
         }
         __syncthreads();
-        float2 v[16];
+        float2 v[R];
 #pragma unroll
-        for (int n2 = 0; n2 < 16; n2++) v[n2] = ex[t + 16 * n2];
+        for (int n2 = 0; n2 < R; n2++) v[n2] = ex[t + 16 * n2];
         __syncthreads();
-        fft256_16<1>(v, t, ex, tw);
+        fft_m<R, 1>(v, t, ex, tw);
 #pragma unroll
-        for (int k1 = 0; k1 < 16; k1++) {
-            const int my = t + 16 * k1, py = my - off;
-            if (py >= 0 && py < N) {
-                const float2 z = v[fft16_slot(k1)];
-                const int mx = c0 + 2 * f + off;
-                so[py * 33 + 2 * f] = chk(mx + my) * z.x * scale;
-                so[py * 33 + 2 * f + 1] = chk(mx + 1 + my) * z.y * scale;
+        for (int j = 0; j < R / 16; j++)
+#pragma unroll
+            for (int k1 = 0; k1 < 16; k1++) {
+                const int my = t + 16 * j + R * k1, py = my - off;
+                if (py >= 0 && py < N) {
+                    const float2 z = v[16 * j + fft16_slot(k1)];
+                    const int mx = c0 + 2 * f + off;
+                    so[py * 33 + 2 * f] = chk(mx + my) * z.x * scale;
+                    so[py * 33 + 2 * f + 1] = chk(mx + 1 + my) * z.y * scale;
+                }
             }
-        }
         __syncthreads();
         for (int idx = tid; idx < N * 32; idx += 256) {
             const int py = idx >> 5, col = idx & 31, px = c0 + col;
@@ -1421,21 +1501,23 @@ __global__ __launch_bounds__(256) void k_inv_cols256(const float2 *__restrict__ 
     }
 }
 
-// Images -> C[a][ky][px] (ky in [0, 129)): 32 image columns per workgroup, two
-// zero-padded real columns per complex transform along y, split into the two
-// half spectra; input through LDS from whole row segments.  Grid
+// Images -> C[a][ky][px] (ky in [0, M/2 + 1)): 32 image columns per workgroup,
+// two zero-padded real columns per complex transform along y, split into the
+// two half spectra; input through LDS from whole row segments.  Grid
 // (ceil(N / 32), images).
-__global__ __launch_bounds__(256) void k_fwd_cols256(const float *__restrict__ imgs, float2 *__restrict__ tmp,
-                                                     int n, int N)
+template <int R>
+__global__ __launch_bounds__(256) void k_fwd_cols(const float *__restrict__ imgs, float2 *__restrict__ tmp,
+                                                  int n, int N)
 {
-    constexpr int M = 256, H = 129;
-    __shared__ float2 sx[16 * FFB];
-    __shared__ float2 tw[256];
-    load_tw256(tw);
+    using G = FftGeom<R>;
+    constexpr int M = G::M, H = G::H;
+    __shared__ float2 sx[16 * G::FFB];
+    __shared__ float2 tw[M];
+    load_tw<R>(tw);
     const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
     const int off = (M - N) / 2;
     const int c0 = 32 * blockIdx.x;
-    float2 *ex = sx + f * FFB;
+    float2 *ex = sx + f * G::FFB;
     float *sf = reinterpret_cast<float *>(sx);
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         __syncthreads();
@@ -1444,22 +1526,24 @@ __global__ __launch_bounds__(256) void k_fwd_cols256(const float *__restrict__ i
             const int my = idx >> 5, col = idx & 31, px = c0 + col, py = my - off;
             float x = 0.f;
             if (px < N && py >= 0 && py < N) x = chk(px + off + my) * imgs[((size_t)a * N + py) * N + px];
-            sf[((col >> 1) * FFB + my) * 2 + (col & 1)] = x;
+            sf[((col >> 1) * G::FFB + my) * 2 + (col & 1)] = x;
         }
         __syncthreads();
-        float2 v[16];
+        float2 v[R];
 #pragma unroll
-        for (int n2 = 0; n2 < 16; n2++) v[n2] = ex[t + 16 * n2];
+        for (int n2 = 0; n2 < R; n2++) v[n2] = ex[t + 16 * n2];
         __syncthreads();
-        fft256_16<-1>(v, t, ex, tw);
+        fft_m<R, -1>(v, t, ex, tw);
 #pragma unroll
-        for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * k1] = v[fft16_slot(k1)];
+        for (int j = 0; j < R / 16; j++)
+#pragma unroll
+            for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * j + R * k1] = v[16 * j + fft16_slot(k1)];
         __syncthreads();
         // C_even[ky] = (Z[ky] + conj Z[M-ky]) / 2, C_odd[ky] = (Z[ky] - conj Z[M-ky]) / (2i)
         for (int idx = tid; idx < H * 32; idx += 256) {
             const int ky = idx >> 5, col = idx & 31, px = c0 + col;
             if (px >= N) continue;
-            const float2 *zf = sx + (col >> 1) * FFB;
+            const float2 *zf = sx + (col >> 1) * G::FFB;
             const float2 z = zf[ky], zm = zf[(M - ky) & (M - 1)];
             tmp[((size_t)a * H + ky) * N + px] = (col & 1)
                 ? make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x))
@@ -1468,46 +1552,50 @@ __global__ __launch_bounds__(256) void k_fwd_cols256(const float *__restrict__ i
     }
 }
 
-// C -> CAS images: per spectrum row ky in [0, 129) the transform along x gives
-// the full row F[ky][.]; CAS row ky and (Hermitian) row M - ky are written
-// whole.  Grid (9 row groups of 16, images).
-__global__ __launch_bounds__(256) void k_fwd_rows256(const float2 *__restrict__ tmp, float *__restrict__ cas,
-                                                     int n, int N, float scale, int zeroNyq)
+// C -> CAS images: per spectrum row ky in [0, M/2 + 1) the transform along x
+// gives the full row F[ky][.]; CAS row ky and (Hermitian) row M - ky are
+// written whole.  Grid (row groups of 16, images).
+template <int R>
+__global__ __launch_bounds__(256) void k_fwd_rows(const float2 *__restrict__ tmp, float *__restrict__ cas,
+                                                  int n, int N, float scale, int zeroNyq)
 {
-    constexpr int M = 256, H = 129;
-    __shared__ float2 sx[16 * FFB];
-    __shared__ float2 tw[256];
-    load_tw256(tw);
+    using G = FftGeom<R>;
+    constexpr int M = G::M, H = G::H;
+    __shared__ float2 sx[16 * G::FFB];
+    __shared__ float2 tw[M];
+    load_tw<R>(tw);
     const int tid = threadIdx.x, f = tid >> 4, t = tid & 15;
     const int off = (M - N) / 2;
     const int ky = 16 * blockIdx.x + f;
     const bool rowOk = ky <= M / 2;
-    float2 *ex = sx + f * FFB;
+    float2 *ex = sx + f * G::FFB;
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         const float2 *crow = tmp + ((size_t)a * H + (rowOk ? ky : 0)) * N;
-        float2 v[16];
+        float2 v[R];
 #pragma unroll
-        for (int n2 = 0; n2 < 16; n2++) {
+        for (int n2 = 0; n2 < R; n2++) {
             const int px = t + 16 * n2 - off;
             v[n2] = (rowOk && px >= 0 && px < N) ? crow[px] : make_float2(0.f, 0.f);
         }
         __syncthreads();
-        fft256_16<-1>(v, t, ex, tw);
+        fft_m<R, -1>(v, t, ex, tw);
         if (rowOk) {
             float *o = cas + (size_t)a * M * M;
             const bool mirror = ky != 0 && ky != M / 2;
 #pragma unroll
-            for (int k1 = 0; k1 < 16; k1++) {
-                const int kx = t + 16 * k1;
-                const float2 F = v[fft16_slot(k1)];
-                const float sd = chk(kx + ky);
-                o[ky * M + kx] = (zeroNyq && (kx == 0 || ky == 0)) ? 0.f : (sd * F.x + sd * F.y) * scale;
-                if (mirror) {
-                    const int x = (M - kx) & (M - 1);  // CAS[M - ky][x] from conj F[ky][(M - x) % M]
-                    const float sm = chk(x + ky);
-                    o[(M - ky) * M + x] = (zeroNyq && x == 0) ? 0.f : (sm * F.x - sm * F.y) * scale;
+            for (int j = 0; j < R / 16; j++)
+#pragma unroll
+                for (int k1 = 0; k1 < 16; k1++) {
+                    const int kx = t + 16 * j + R * k1;
+                    const float2 F = v[16 * j + fft16_slot(k1)];
+                    const float sd = chk(kx + ky);
+                    o[ky * M + kx] = (zeroNyq && (kx == 0 || ky == 0)) ? 0.f : (sd * F.x + sd * F.y) * scale;
+                    if (mirror) {
+                        const int x = (M - kx) & (M - 1);  // CAS[M - ky][x] from conj F[ky][(M - x) % M]
+                        const float sm = chk(x + ky);
+                        o[(M - ky) * M + x] = (zeroNyq && x == 0) ? 0.f : (sm * F.x - sm * F.y) * scale;
+                    }
                 }
-            }
         }
     }
 }
@@ -2042,17 +2130,35 @@ hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hi
     return hipGetLastError();
 }
 
-bool fused_chain_supported(int M, int N) { return M == 256 && N >= 1 && N <= M; }
+bool fused_chain_supported(int M, int N) { return (M == 256 || M == 512) && N >= 1 && N <= M; }
+
+template <int R>
+static void fused_inverse(const float *cas, float2 *tmp, float *imgs, int N, int n, float scale, hipStream_t s)
+{
+    constexpr int H = 8 * R + 1;
+    const unsigned ny = (unsigned)std::min(n, 65535);
+    hipLaunchKernelGGL(k_inv_rows<R>, dim3((H + 15) / 16, ny), dim3(256), 0, s, cas, tmp, n, N);
+    hipLaunchKernelGGL(k_inv_cols<R>, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, tmp, imgs, n, N,
+                       scale);
+}
+
+template <int R>
+static void fused_forward(const float *imgs, float2 *tmp, float *cas, int N, int n, float scale, int zeroNyq,
+                          hipStream_t s)
+{
+    constexpr int H = 8 * R + 1;
+    const unsigned ny = (unsigned)std::min(n, 65535);
+    hipLaunchKernelGGL(k_fwd_cols<R>, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, imgs, tmp, n, N);
+    hipLaunchKernelGGL(k_fwd_rows<R>, dim3((H + 15) / 16, ny), dim3(256), 0, s, tmp, cas, n, N, scale, zeroNyq);
+}
 
 hipError_t launch_fused_cas_to_images(const float *cas, float2 *tmp, float *imgs, int M, int N, int n,
                                       float scale, hipStream_t s)
 {
     if (n <= 0) return hipSuccess;
     if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
-    const unsigned ny = (unsigned)std::min(n, 65535);
-    hipLaunchKernelGGL(k_inv_rows256, dim3(9, ny), dim3(256), 0, s, cas, tmp, n, N);
-    hipLaunchKernelGGL(k_inv_cols256, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, tmp, imgs, n, N,
-                       scale);
+    if (M == 256) fused_inverse<16>(cas, tmp, imgs, N, n, scale, s);
+    else fused_inverse<32>(cas, tmp, imgs, N, n, scale, s);
     return hipGetLastError();
 }
 
@@ -2061,9 +2167,8 @@ hipError_t launch_fused_images_to_cas(const float *imgs, float2 *tmp, float *cas
 {
     if (n <= 0) return hipSuccess;
     if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
-    const unsigned ny = (unsigned)std::min(n, 65535);
-    hipLaunchKernelGGL(k_fwd_cols256, dim3((unsigned)((N + 31) / 32), ny), dim3(256), 0, s, imgs, tmp, n, N);
-    hipLaunchKernelGGL(k_fwd_rows256, dim3(9, ny), dim3(256), 0, s, tmp, cas, n, N, scale, zeroNyq);
+    if (M == 256) fused_forward<16>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
+    else fused_forward<32>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
     return hipGetLastError();
 }
 
