@@ -18,6 +18,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -60,6 +61,32 @@ def measured_traffic(path: str, n: int, kernel: str, chunk: int):
         if line.strip().startswith(key):
             return int(float(line.split(":", 1)[1].split()[0]) * chunk)
     return None
+
+
+def profile_value(path: str, key: str):
+    """A 'key: value' line of a committed profile summary, or None."""
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.strip().startswith(key):
+            return float(line.split(":", 1)[1].split()[0])
+    return None
+
+
+def run_peak_probe():
+    """Measured VALU / random-LDS / copy peaks of this GPU (multigpugridder_amd/peak_probe,
+    built with the library); None when the probe is not built or fails."""
+    exe = os.path.join(ROOT, "multigpugridder_amd", "peak_probe")
+    if not os.path.exists(exe):
+        return None
+    try:
+        r = subprocess.run([exe], capture_output=True, text=True, timeout=120)
+    except (OSError, subprocess.TimeoutExpired):
+        return None
+    if r.returncode != 0:
+        return None
+    lines = [x for x in r.stdout.splitlines() if x.startswith("{")]
+    return json.loads(lines[-1]) if lines else None
 
 
 def cpu_baseline(n: int, p_sample: int, seed: int = 1):
@@ -292,6 +319,9 @@ def main():
                          "kernel that scales rocprofv3 FETCH_SIZE / WRITE_SIZE readings)")
     ap.add_argument("--traffic-file", default=os.path.join(ROOT, "profiles", "bp_kernel_traffic_n128.md"),
                     help="profile summary holding the measured HBM bytes per back-projection direction")
+    ap.add_argument("--issue-file", default=os.path.join(ROOT, "profiles", "issue_kernels_n128.md"),
+                    help="profile summary holding the projectors' VALU instructions per direction")
+    ap.add_argument("--no-peak-probe", action="store_true", help="skip the VALU / LDS / copy peak probe")
     ap.add_argument("--serve-requests", type=int, default=150,
                     help="requests per serving measurement (0: skip the serving leg)")
     ap.add_argument("--serve-lambdas", default="0.8",
@@ -466,6 +496,30 @@ def main():
             "avg_launch_ms": round(launch_ms, 4),
         },
     }
+    # Issue roofline: the projectors are gathers bound by instruction issue, so
+    # their VALU rate (rocprof instruction count per direction x directions per
+    # launch / live launch time) is set beside the measured VALU issue peak.
+    probe = None if args.no_peak_probe else run_peak_probe()
+    ncu = torch.cuda.get_device_properties(dev).multi_processor_count
+    kernels = {}
+    for name, lms in (("bp", bp_launch_ms), ("fp", fp_launch_ms)):
+        vpd = profile_value(args.issue_file, f"valu_per_direction {name} N={n}:")
+        if vpd is None:
+            continue
+        rate = vpd * chunk / ncu / (lms * 1e6)
+        kernels[name] = {"valu_per_launch": vpd * chunk, "avg_launch_ms": round(lms, 4),
+                         "achieved": round(rate, 4)}
+        if probe:
+            kernels[name]["frac"] = round(rate / probe["valu_add_wave_inst_per_cu_ns"], 4)
+    if kernels:
+        dom = kernels.get(dominant, {})
+        out["issue_roofline"] = {
+            "kernel": out["roofline"]["kernel"], "bound": "valu_issue",
+            "achieved": dom.get("achieved"), "peak": probe["valu_add_wave_inst_per_cu_ns"] if probe else None,
+            "unit": "wave64 VALU instructions per CU per ns", "frac": dom.get("frac"),
+            "kernels": kernels, "probe": probe,
+            "source": "VALU instructions per direction: rocprofv3 SQ_INSTS_VALU (profiles/issue_kernels_n128.md); "
+                      "peak: peak_probe on this GPU (independent FP32 adds)"}
     if args.serve_requests > 0:
         del cas_bufs, imgs, bp_vol
         torch.cuda.empty_cache()
