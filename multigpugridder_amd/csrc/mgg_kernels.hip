@@ -623,6 +623,7 @@ struct BPArgs {
     float maskR, kerScale, kerCenter, fW, hw;
     int convW, bricksX, bricksXY;
     int maskD2;  // largest integer d2 with sqrtf((float)d2) <= maskR (-1: none)
+    int posRound;  // bp_fast_kernel: in-mask voxels of orthonormal frames project to fi, fj >= 0.5
     const int *order;  // bp_fast_kernel: brick id per work item, heaviest first
     unsigned perXcd, totalWork;
 };
@@ -898,7 +899,7 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
     // the unfused lookup; the brick's smallest coordinate on the axis decides.
     auto formOf = [](int o) { return o >= BP_FORM_A_MIN ? 2 : (o >= BP_FORM_B_MIN ? 1 : 0); };
     const int fx = formOf(ox), fy = formOf(oy), fz = formOf(oz);
-    const int brickForm = (fx && fy && fz) ? fx + 2 * (fy - 1) + 4 * (fz - 1) : 0;
+    const int brickForm = (p.posRound && fx && fy && fz) ? fx + 2 * (fy - 1) + 4 * (fz - 1) : 0;
 
     const float cvf = (float)cv, cif = (float)ci;
     const float bc = 0.5f * (BB - 1);
@@ -1003,16 +1004,19 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                     const float dvi = (float)vi - cv, dvj = (float)vj - cv, dvk = (float)vk - cv;
                     const float fi = dvi * u0 + dvj * u1 + dvk * u2 + cif;
                     const float fj = dvi * v0 + dvj * v1 + dvk * v2 + cif;
-                    const int I = (int)roundf(fi), J = (int)roundf(fj);
-                    const float fvi = (float)vi, fvj = (float)vj, fvk = (float)vk;
-                    // Form constants (used only by the fused variants).
                     constexpr int KF = decltype(formT)::value;
+                    // The fused variants run only when posRound holds (fi, fj >= 0.5),
+                    // where floor(x + 0.5) is roundf(x) and one instruction.
+                    const int I = KF > 0 ? cvt_rpi(fi) : (int)roundf(fi);
+                    const int J = KF > 0 ? cvt_rpi(fj) : (int)roundf(fj);
+                    const float fvi = (float)vi, fvj = (float)vj, fvk = (float)vk;
+                    // Form constants (used only by the fused variants; exact).
                     constexpr int FX = KF > 0 ? 1 + ((KF - 1) & 1) : 0;
                     constexpr int FY = KF > 0 ? 1 + (((KF - 1) >> 1) & 1) : 0;
                     constexpr int FZ = KF > 0 ? 1 + (((KF - 1) >> 2) & 1) : 0;
-                    const float cX = (FX == 2 ? 250.0f : 0.0f) - 125.0f * fvi;
-                    const float cY = (FY == 2 ? 250.0f : 0.0f) - 125.0f * fvj;
-                    const float cZ = (FZ == 2 ? 250.0f : 0.0f) - 125.0f * fvk;
+                    const float cX = FX == 2 ? fmaf(-125.0f, fvi, 250.0f) : -125.0f * fvi;
+                    const float cY = FY == 2 ? fmaf(-125.0f, fvj, 250.0f) : -125.0f * fvj;
+                    const float cZ = FZ == 2 ? fmaf(-125.0f, fvk, 250.0f) : -125.0f * fvk;
                     float Ax[5], Ay[5], Az[5];
 #pragma unroll
                     for (int s5 = 0; s5 < 5; s5++) {
@@ -2073,6 +2077,11 @@ hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
     const bool fast = bp_uses_fast_path(g);
     a.maskD2 = mask_d2(g.maskRadius, g.V);
+    // In-mask voxel: |v - c| <= maskR; orthonormal frame (to 1e-4): |u|, |v| <=
+    // 1.00005, and the FP32 evaluation of fi adds < 1e-3.  Then fi, fj >=
+    // M/2 - 1.00005 maskR - 1e-3, which must be >= 0.5 for floor(fi + 0.5) to be
+    // roundf(fi) (for 0.25 <= x < 0.5 an FP32 x + 0.5 can round up to 1).
+    a.posRound = ((double)(g.M / 2) - 1.00005 * (double)g.maskRadius - 1e-3 >= 0.5) ? 1 : 0;
     const int ex = fast ? BB : BPX, ey = fast ? BB : BPY, ez = fast ? BB : BPZ;
     a.bricksX = (g.V + ex - 1) / ex;
     const int bricksY = (g.V + ey - 1) / ey;
