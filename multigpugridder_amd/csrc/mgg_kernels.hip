@@ -228,8 +228,9 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
 // (bounded from the tile's four corner positions, about 30 voxels per pixel),
 // reading each box row as unaligned 16-byte segments, and gathers the 125 taps
 // from LDS instead of issuing 25 scattered row loads per pixel to the texture
-// path.  A tile whose box exceeds the per-wave LDS budget (unusual geometry
-// only) gathers from global memory.  The gather is arranged as
+// path.  A tile whose box exceeds the per-wave LDS budget is staged and
+// gathered as two 8x4 halves, and a half whose box still does not fit gathers
+// from global memory.  The gather is arranged as
 // acc += wk[c] * sum_ab (wi[a]*wj[b]) * vol, i.e. the reference's product
 // (wi*wj)*wk regrouped: rounding decisions are unchanged, only FP32
 // accumulation rounding differs.
@@ -381,17 +382,18 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         }
         if (PP * Dz > FPBOX) return false;
 
-        // Stage: lane -> (row slot, segment); 64 / segs rows per pass.
-        const int rowsPer = 64 / segs;
-        const int rIn = ln / segs, sg = ln - rIn * segs;
-        const int nRows = Dy * Dz;
+        // Stage.  A plane of the box holds P = Dy * segs row segments.  With
+        // P <= 64 one pass covers G = 64 / P planes (lane -> plane offset pl,
+        // row ey, segment sg), else G = 1 and a plane takes ceil(P / 64) lane
+        // passes; each lane then steps through its planes with one add for
+        // the volume offset and one for the LDS pointer.  The lane -> task
+        // divisions use the reciprocal: (q + 0.5) / P is at least 0.5 / P
+        // away from an integer, far more than its error for q, P <= 1024.
         const int VV = V * V, V3i = (int)V3;
-        // row -> (ez, ey) = (row / Dy, row % Dy), stepped by rowsPer per pass
-        const int qs = rowsPer / Dy, rs = rowsPer - qs * Dy;
-        const int ez = rIn / Dy, ey = rIn - ez * Dy;
-        const int lin = (bz0 + ez) * VV + (by0 + ey) * V + (bx0 + 4 * sg);
-        const int linStep = qs * VV + rs * V, linWrap = VV - Dy * V;
-        const int dStep = qs * PP + rs * RP, dWrap = PP - Dy * RP;
+        const int P = Dy * segs;
+        const float rP = __builtin_amdgcn_rcpf((float)P), rSegs = __builtin_amdgcn_rcpf((float)segs);
+        const int G = P <= 64 ? __builtin_amdgcn_readfirstlane((int)(64.5f * rP)) : 1;
+        const int nLanePass = (P + 63) >> 6;
         // The reference bounds only the linear index (gpuForwardProject.cu:90):
         // when the box's whole index range is inside the volume no element
         // needs a check.  lin grows with ez, ey and the segment, so the range
@@ -399,52 +401,56 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         const long long linLo = (long long)bz0 * VV + (long long)by0 * V + bx0;
         const long long linHi = (long long)(bz0 + Dz - 1) * VV + (long long)(by0 + Dy - 1) * V + bx0 + RP - 1;
         const bool boxIn = linLo >= 0 && linHi < V3;
-        // Four passes per batch: all loads of a batch are in flight before
-        // the first LDS store waits on them.  The checked and unchecked
-        // variants are separate loops (boxIn is wave-uniform).
+        // Four planes per batch: all loads of a batch are in flight before the
+        // first LDS store waits on them.  The checked and unchecked variants
+        // are separate loops (boxIn is wave-uniform); the unchecked one
+        // addresses the volume by a 32-bit byte offset from its base
+        // (V^3 * sizeof(T) < 2^32 is required above).
         constexpr int SB = 4;
-        auto stage = [&](auto inBox) {
-            int row = rIn, e_y = ey, l = lin;
-            int d = ez * PP + ey * RP + 4 * sg;  // LDS offset of the lane's segment
-            for (int r0 = 0; r0 < nRows; r0 += SB * rowsPer) {
-                float v4[SB][4];
-                int dst[SB];
+        const int lStep = G * VV, dStep = G * PP;
+        for (int c = 0; c < nLanePass; c++) {
+            const int q = ln + 64 * c;
+            const int pl = (int)(((float)q + 0.5f) * rP);
+            const int pq = q - pl * P;
+            const int ey = (int)(((float)pq + 0.5f) * rSegs);
+            const int sg = pq - ey * segs;
+            if (pl >= G) continue;
+            const int lin0 = (bz0 + pl) * VV + (by0 + ey) * V + (bx0 + 4 * sg);
+            float *d0 = box + pl * PP + ey * RP + 4 * sg;
+            auto stage = [&](auto inBox) {
+                int l = lin0;
+                float *dp = d0;
+                for (int ez = pl; ez < Dz; ez += SB * G) {
+                    float v4[SB][4];
 #pragma unroll
-                for (int q = 0; q < SB; q++) {
-                    dst[q] = -1;
-                    if (rIn < rowsPer && row < nRows) {
-                        if (decltype(inBox)::value) {
-                            load4<L>(vol + l, v4[q]);
-                        } else {
+                    for (int k = 0; k < SB; k++) {
+                        if (ez + k * G < Dz) {
+                            const int lk = l + k * lStep;
+                            if (decltype(inBox)::value) {
+                                load4<L>((const T *)((const char *)vol + (unsigned)lk * (unsigned)sizeof(T)), v4[k]);
+                            } else {
 #pragma unroll
-                            for (int e = 0; e < 4; e++)
-                                v4[q][e] = (l + e >= 0 && l + e < V3i) ? L::one(vol + l + e) : 0.f;
+                                for (int e = 0; e < 4; e++)
+                                    v4[k][e] = (lk + e >= 0 && lk + e < V3i) ? L::one(vol + lk + e) : 0.f;
+                            }
                         }
-                        dst[q] = d;
                     }
-                    row += rowsPer;
-                    e_y += rs;
-                    l += linStep;
-                    d += dStep;
-                    if (e_y >= Dy) {
-                        e_y -= Dy;
-                        l += linWrap;
-                        d += dWrap;
-                    }
-                }
 #pragma unroll
-                for (int q = 0; q < SB; q++) {
-                    if (dst[q] >= 0) {
-                        f4 w4 = {v4[q][0], v4[q][1], v4[q][2], v4[q][3]};
-                        *(f4 *)(box + dst[q]) = w4;
+                    for (int k = 0; k < SB; k++) {
+                        if (ez + k * G < Dz) {
+                            f4 w4 = {v4[k][0], v4[k][1], v4[k][2], v4[k][3]};
+                            *(f4 *)(dp + k * dStep) = w4;
+                        }
                     }
+                    l += SB * lStep;
+                    dp += SB * dStep;
                 }
+            };
+            if (boxIn) {
+                stage(std::true_type{});
+            } else {
+                stage(std::false_type{});
             }
-        };
-        if (boxIn) {
-            stage(std::true_type{});
-        } else {
-            stage(std::false_type{});
         }
         // 16-byte gather: each 5-tap row is read as the two aligned 16-byte
         // units that contain it, [xo - o, xo - o + 8) with o = xo & 3, and the
@@ -515,9 +521,9 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         return true;
     };
 
-    // The whole 8x8 tile; if its box is too large (3% of tiles for the grid
-    // directions at N=128), its two 8x4 halves one after the other; a half
-    // that still does not fit gathers from global memory.
+    // The whole 8x8 tile; if its box is too large (27% of the tiles for the
+    // grid directions at N=128), its two 8x4 halves one after the other; a
+    // half that still does not fit gathers from global memory.
     bool done = stageAndGather(0, FPT - 1, true);
     if (!done) {
         const bool top = (ln >> 3) < FPT / 2;
