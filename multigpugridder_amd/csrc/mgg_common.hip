@@ -55,19 +55,18 @@ void raise(int code, const char *fmt, ...)
 
 
 // ---------------------------------------------------------------------------
-// rocFFT plans.  A plan plus its work buffer is bound to one stream so that
-// concurrent streams never share a work buffer.
+// rocFFT plans.  A plan and its execution info are bound to one stream, and
+// all plans of a stream share one work buffer (work on a stream runs in
+// order), grown when a new plan needs more.  Plans without a buffer of their
+// own are small, so the cache holds every batch size fft2d_batched uses.
 // ---------------------------------------------------------------------------
 std::once_flag g_fft_once;
 
 struct FftPlan {
     rocfft_plan plan = nullptr;
     rocfft_execution_info info = nullptr;
-    void *work = nullptr;
-    int device = -1;
     ~FftPlan()
     {
-        if (work) (void)hipFree(work);
         if (info) rocfft_execution_info_destroy(info);
         if (plan) rocfft_plan_destroy(plan);
     }
@@ -75,12 +74,19 @@ struct FftPlan {
 
 // kind: 0 = 2-D R2C, 1 = 2-D C2R, 2 = 3-D R2C, 3 = 3-D C2R
 using PlanKey = std::tuple<int, int, int, size_t, hipStream_t>;
+using StreamKey = std::pair<int, hipStream_t>;
+
+struct StreamWork {
+    void *buf = nullptr;
+    size_t bytes = 0;
+};
 
 struct PlanCache {
     std::mutex mu;
     std::map<PlanKey, std::unique_ptr<FftPlan>> plans;
     std::vector<PlanKey> order;  // insertion order, for eviction
-    static constexpr size_t kMaxPlans = 64;
+    std::map<StreamKey, StreamWork> work;
+    static constexpr size_t kMaxPlans = 1024;
 
     FftPlan *get(int device, int kind, int m, size_t batch, hipStream_t stream)
     {
@@ -93,12 +99,10 @@ struct PlanCache {
             // The oldest plan may still be queued on its stream: wait for it.
             PlanKey old = order.front();
             order.erase(order.begin());
-            hipStream_t os = std::get<4>(old);
-            (void)hipStreamSynchronize(os);
+            (void)hipStreamSynchronize(std::get<4>(old));
             plans.erase(old);
         }
         auto p = std::make_unique<FftPlan>();
-        p->device = device;
         const size_t len2[2] = {(size_t)m, (size_t)m};
         const size_t len3[3] = {(size_t)m, (size_t)m, (size_t)m};
         const bool threeD = kind >= 2;
@@ -110,20 +114,33 @@ struct PlanCache {
         size_t wbytes = 0;
         FFT_TRY(rocfft_plan_get_work_buffer_size(p->plan, &wbytes));
         FFT_TRY(rocfft_execution_info_create(&p->info));
-        if (wbytes) {
-            hipError_t e = hipMalloc(&p->work, wbytes);
+        FFT_TRY(rocfft_execution_info_set_stream(p->info, stream));
+        StreamWork &w = work[StreamKey{device, stream}];
+        if (wbytes > w.bytes) {
+            // The stream's queued transforms may still use the old buffer.
+            if (w.buf) {
+                HIP_TRY(hipStreamSynchronize(stream));
+                (void)hipFree(w.buf);
+                w.buf = nullptr;
+                w.bytes = 0;
+            }
+            hipError_t e = hipMalloc(&w.buf, wbytes);
             if (e != hipSuccess)
                 raise(MGG_ERR_OUT_OF_MEMORY, "rocFFT work buffer (%zu bytes): %s", wbytes,
                       hipGetErrorString(e));
-            FFT_TRY(rocfft_execution_info_set_work_buffer(p->info, p->work, wbytes));
+            w.bytes = wbytes;
+            for (auto &kv : plans)
+                if (std::get<0>(kv.first) == device && std::get<4>(kv.first) == stream)
+                    FFT_TRY(rocfft_execution_info_set_work_buffer(kv.second->info, w.buf, w.bytes));
         }
-        FFT_TRY(rocfft_execution_info_set_stream(p->info, stream));
+        if (w.bytes) FFT_TRY(rocfft_execution_info_set_work_buffer(p->info, w.buf, w.bytes));
         FftPlan *raw = p.get();
         plans.emplace(key, std::move(p));
         order.push_back(key);
         return raw;
     }
 
+    // Called for a stream that is about to be destroyed (its work is done).
     void drop_stream(hipStream_t s)
     {
         std::lock_guard<std::mutex> lk(mu);
@@ -131,6 +148,14 @@ struct PlanCache {
             if (std::get<4>(*it) == s) {
                 plans.erase(*it);
                 it = order.erase(it);
+            } else {
+                ++it;
+            }
+        }
+        for (auto it = work.begin(); it != work.end();) {
+            if (it->first.second == s) {
+                if (it->second.buf) (void)hipFree(it->second.buf);
+                it = work.erase(it);
             } else {
                 ++it;
             }
@@ -160,12 +185,18 @@ void fft_exec(int kind, int m, size_t batch, void *in, void *out, hipStream_t s)
 }
 
 
+// Batches of kFftBatch (a power of two), then the remainder as a sum of powers
+// of two: at most log2(kFftBatch) + 1 plan sizes exist per transform shape and
+// stream, where a plan per remainder would mean a new plan (and work buffer)
+// for most call sizes.
 void fft2d_batched(int kind, int m, size_t n, void *in, size_t inStride, void *out,
                    size_t outStride, hipStream_t s)
 {
-    for (size_t a = 0; a < n; a += kFftBatch) {
-        const size_t b = std::min(kFftBatch, n - a);
+    for (size_t a = 0; a < n;) {
+        size_t b = std::min(kFftBatch, n - a);
+        b = size_t(1) << (63 - __builtin_clzll((unsigned long long)b));  // largest power of two <= b
         fft_exec(kind, m, b, (char *)in + a * inStride, (char *)out + a * outStride, s);
+        a += b;
     }
 }
 
