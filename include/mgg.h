@@ -15,7 +15,7 @@
  *   directions   axes[9*a + 0..8] = [u; v; n]         9 floats per direction:
  *                image x-axis, image y-axis, projection normal (volume frame).
  *
- * Three layers:
+ * Four layers:
  *   1. the gridder object (mgg_create ... mgg_reconstruct_volume): the
  *      reference MultiGPUGridder / Matlab class surface.  Host arrays are
  *      borrowed (never copied at set time, never freed), results are written
@@ -147,7 +147,9 @@ int mgg_reset_volume(mgg_gridder *g);
 /* All pointers are device pointers on the current device; stream is a
  * hipStream_t (NULL = the null stream).  Calls are asynchronous on stream.
  * ker is the device copy of the Kaiser-Bessel table (ker_size floats,
- * half width ker_hwidth). */
+ * half width ker_hwidth).  Threads may call operators concurrently on
+ * different streams; calls on one stream must come from one thread at a time
+ * (the image and volume chains keep per-stream FFT plans and work memory). */
 
 /* Forward projection (Projection/gpuForwardProject.cu:4-101): writes n_axes
  * CAS images of m x m (0 outside the mask disc).  cas_volume holds v^3

@@ -222,6 +222,7 @@ struct mgg_server {
     std::mutex pcMu;
     std::map<int, std::unique_ptr<DevBuf>> precomp;  // per M, guarded by pcMu
     hipStream_t adminStream = nullptr;
+    std::mutex adminMu;  // one volume build at a time on adminStream (its FFT plans and work memory)
     std::vector<std::unique_ptr<Pipeline>> pipes;
 
     std::mutex mu;
@@ -786,6 +787,7 @@ int mgg_server_add_volume(mgg_server *s, const float *volume, int volume_size, f
         }
         int code = MGG_OK;
         try {
+            std::lock_guard<std::mutex> ak(s->adminMu);
             Carver c{(char *)tmpMem};
             float *dVol = c.take<float>(sizeof(float) * N * N * N);
             float *dReal3 = c.take<float>(sizeof(float) * M * M * M);
