@@ -121,6 +121,7 @@ struct FPArgs {
     float maskR, kerScale, kerCenter, fW;
     int convW, tilesPerRow, tilesPerImg;
     unsigned perXcd, totalWork;
+    int maskD2;  // fp_lds_kernel: largest integer d2 with sqrtf((float)d2) <= maskR (-1: none)
 };
 
 template <int DT, bool W2>
@@ -252,6 +253,15 @@ __device__ __forceinline__ void wave_lds_sync_fp()
     __builtin_amdgcn_wave_barrier();
 }
 
+// floor(x + 0.5) in one instruction; equals roundf(x) for x >= 0.5 (below,
+// an FP32 x + 0.5 can round up to the next integer).
+__device__ __forceinline__ int cvt_rpi(float x)
+{
+    int i;
+    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(x));
+    return i;
+}
+
 // 501-entry table lookup (see bp_fast_kernel for the exactness argument),
 // read from the table in global memory (2 KB, cache-resident): the forward
 // kernel does only 15 lookups per pixel, and keeping the table out of LDS
@@ -262,9 +272,8 @@ __device__ __forceinline__ float kb_fast_fp(const float *__restrict__ ker, float
 {
     float s = r * 125.0f;
     s = s + 250.0f;
-    int i;
-    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(s));
-    return ker[min(max(i, 0), 500)];
+    const int i = cvt_rpi(s);
+    return *(const float *)((const char *)ker + (unsigned)min(max(i, 0), 500) * 4u);
 }
 
 template <typename L>
@@ -309,8 +318,9 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     const int M = p.M, V = p.V, ci = M / 2, cv = V / 2;
     const bool pix = i < M && j < M;
 
-    const float r = sqrtf((float)(i - ci) * (i - ci) + (j - ci) * (j - ci));
-    const bool active = pix && r <= p.maskR;
+    // sqrtf(d2) <= maskR for the exact integer d2, as one integer compare
+    // (sqrtf is monotone; the host finds the largest passing d2).
+    const bool active = pix && (i - ci) * (i - ci) + (j - ci) * (j - ci) <= p.maskD2;
     float *out = p.out + (size_t)a * M * M + (size_t)j * M + i;
     if (__ballot(active) == 0ull) {
         if (pix) *out = 0.f;
@@ -558,8 +568,9 @@ __global__ __launch_bounds__(kThreads) void fp_lds_kernel(FPArgs p)
         for (int t = threadIdx.x; t < p.K; t += kThreads) sker[t] = p.ker[t];
         __syncthreads();
     }
-    // Each wave takes work items wv + 4*blockIdx.x, + 4*gridDim.x, ...
-    const int wv = threadIdx.x >> 6, ln = threadIdx.x & 63;
+    // Each wave takes work items wv + 4*blockIdx.x, + 4*gridDim.x, ...  (wave
+    // uniform, so the work index and its divisions stay scalar)
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
     for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
         fp_lds_tile<DT, FAST>(p, sker, sbox[wv], work, ln);
 }
@@ -810,13 +821,6 @@ constexpr int KFAST = 501;
 constexpr int KTOT = KFAST + 2 * KPAD; // padded table length
 constexpr int QCAP = 1024;             // per-wave queue ring (>= BV + 64, power of 2)
 constexpr int BPCH = 128;              // directions culled per pass (queue entries hold 9-bit voxel + 7-bit slot)
-
-__device__ __forceinline__ int cvt_rpi(float x)
-{
-    int i;
-    asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(i) : "v"(x));
-    return i;
-}
 
 __device__ __forceinline__ float kb_fast(const float *__restrict__ sk, float r)
 {
@@ -1971,6 +1975,7 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
         const bool fast = a.kerScale == 125.0f && a.kerCenter == 250.0f && g.K == 501;
         a.tilesPerRow = (g.M + FPT - 1) / FPT;
         a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
+        a.maskD2 = mask_d2(g.maskRadius, g.V);
         const unsigned long long work = (unsigned long long)a.tilesPerImg * nAxes;
         if (work > 0xffffffffull / 2) return hipErrorInvalidValue;
         a.totalWork = (unsigned)work;
