@@ -1227,12 +1227,13 @@ __global__ void k_herm2d_to_cas(const float2 *__restrict__ herm, float *__restri
 }
 
 // ---------------------------------------------------------------------------
-// Fused image chains for M = 256 and 512.  The general chains above make five
-// or six HBM passes per image (CAS -> Hermitian half spectrum, two rocFFT
+// Fused image chains for M = 128, 256 and 512.  The general chains above make
+// five or six HBM passes per image (CAS -> Hermitian half spectrum, two rocFFT
 // passes, crop; or the reverse).  These make two each, with the M-point FFT
 // done in LDS as four-step 16 x R (M = 16 R; 16 threads per transform, the
 // R-point and 16-point DFTs in registers, one twiddled exchange through LDS;
-// the 32-point DFT is two 16-point DFTs and a radix-2 step), and are ordered
+// the 32-point DFT is two 16-point DFTs and a radix-2 step, the 8-point DFT
+// two 4-point DFTs and a radix-2 step), and are ordered
 // so that CAS images are read and written as whole rows:
 //   CAS images -> images: (1) k_inv_rows: per spectrum row ky <= M/2 the full
 //   complex row is built from CAS rows ky and M - ky as they are read,
@@ -1347,10 +1348,34 @@ __device__ __forceinline__ void fft32(float2 (&v)[32])
     }
 }
 
+// 8-point DFT in registers: two 4-point DFTs (even and odd inputs) and a
+// radix-2 combination; input and output in natural order.
+template <int S>
+__device__ __forceinline__ void fft8(float2 (&v)[8])
+{
+    float2 e0 = v[0], e1 = v[2], e2 = v[4], e3 = v[6];
+    float2 o0 = v[1], o1 = v[3], o2 = v[5], o3 = v[7];
+    fft4<S>(e0, e1, e2, e3);
+    fft4<S>(o0, o1, o2, o3);
+    constexpr float h = 0.70710678118654752f;  // cos(pi / 4)
+    o1 = cmulw(o1, h, S > 0 ? h : -h);
+    o2 = S > 0 ? make_float2(-o2.y, o2.x) : make_float2(o2.y, -o2.x);
+    o3 = cmulw(o3, -h, S > 0 ? h : -h);
+    v[0] = make_float2(e0.x + o0.x, e0.y + o0.y);
+    v[4] = make_float2(e0.x - o0.x, e0.y - o0.y);
+    v[1] = make_float2(e1.x + o1.x, e1.y + o1.y);
+    v[5] = make_float2(e1.x - o1.x, e1.y - o1.y);
+    v[2] = make_float2(e2.x + o2.x, e2.y + o2.y);
+    v[6] = make_float2(e2.x - o2.x, e2.y - o2.y);
+    v[3] = make_float2(e3.x + o3.x, e3.y + o3.y);
+    v[7] = make_float2(e3.x - o3.x, e3.y - o3.y);
+}
+
 template <int R, int S>
 __device__ __forceinline__ void fft_regs(float2 (&v)[R])
 {
-    if constexpr (R == 16) fft16<S>(v);
+    if constexpr (R == 8) fft8<S>(v);
+    else if constexpr (R == 16) fft16<S>(v);
     else fft32<S>(v);
 }
 
@@ -1358,7 +1383,14 @@ __device__ __forceinline__ void fft_regs(float2 (&v)[R])
 template <int R>
 __device__ __forceinline__ constexpr int fftR_slot(int k)
 {
-    return R == 16 ? fft16_slot(k) : (k < 16 ? fft16_slot(k) : 16 + fft16_slot(k - 16));
+    return R == 8 ? k : (R == 16 ? fft16_slot(k) : (k < 16 ? fft16_slot(k) : 16 + fft16_slot(k - 16)));
+}
+
+// Register of X[t + 16 m] (m < R) after fft_m<R> (see there).
+template <int R>
+__device__ __forceinline__ constexpr int out_slot(int m)
+{
+    return R == 8 ? m : (R == 16 ? fft16_slot(m) : 16 * (m & 1) + fft16_slot(m >> 1));
 }
 
 constexpr int FFP = 17;  // exchange pitch (float2)
@@ -1368,8 +1400,13 @@ template <int R> struct FftGeom {
 };
 
 // M-point DFT (M = 16 R) by the 16 threads t = 0..15 of one transform,
-// four-step 16 x R.  On entry v[n2] holds x[t + 16 n2] (n2 < R); on exit
-// v[16 j + fft16_slot(k1)] holds X[t + 16 j + R k1] (j < R / 16, k1 < 16).
+// four-step 16 x R: R-point DFTs in registers, twiddles, exchange through
+// LDS, 16-point DFTs.  On entry v[n2] holds x[t + 16 n2] (n2 < R); on exit
+// v[out_slot<R>(m)] holds X[t + 16 m] (m < R).  With R >= 16 thread t does the
+// 16-point DFTs of k2 = t + 16 j (outputs X[k2 + R k1]); with R = 8 each of
+// the eight 16-point DFTs is shared by threads k2 and k2 + 8 through one
+// radix-2 decimation in frequency (thread h = t / 8 takes the outputs
+// k1 = 2 m + h, an 8-point DFT of (z[n] + (-1)^h z[n + 8]) w16^(n h)).
 // ex is the transform's FFB-float2 LDS exchange area, tw the M twiddles
 // e^{2 pi i q / M}.  Contains workgroup barriers: every thread of the block
 // must call it.
@@ -1383,14 +1420,27 @@ __device__ __forceinline__ void fft_m(float2 (&v)[R], int t, float2 *ex, const f
         ex[k2 * FFP + t] = cmulw(v[fftR_slot<R>(k2)], w.x, S > 0 ? w.y : -w.y);
     }
     __syncthreads();
+    if constexpr (R == 8) {
+        const int k2 = t & 7, hf = t >> 3;
+        const float sg = hf ? -1.f : 1.f;
 #pragma unroll
-    for (int j = 0; j < R / 16; j++) {
-        float2 u[16];
+        for (int n = 0; n < 8; n++) {
+            const float2 z0 = ex[k2 * FFP + n], z1 = ex[k2 * FFP + n + 8];
+            const float2 d = make_float2(fmaf(sg, z1.x, z0.x), fmaf(sg, z1.y, z0.y));
+            const float2 w = tw[((16 * R) / 16) * n * hf];  // w16^(n h)
+            v[n] = cmulw(d, w.x, S > 0 ? w.y : -w.y);
+        }
+        fft8<S>(v);
+    } else {
 #pragma unroll
-        for (int n1 = 0; n1 < 16; n1++) u[n1] = ex[(t + 16 * j) * FFP + n1];
-        fft16<S>(u);
+        for (int j = 0; j < R / 16; j++) {
+            float2 u[16];
 #pragma unroll
-        for (int q = 0; q < 16; q++) v[16 * j + q] = u[q];
+            for (int n1 = 0; n1 < 16; n1++) u[n1] = ex[(t + 16 * j) * FFP + n1];
+            fft16<S>(u);
+#pragma unroll
+            for (int q = 0; q < 16; q++) v[16 * j + q] = u[q];
+        }
     }
     __syncthreads();  // the exchange area may be reused
 }
@@ -1446,12 +1496,10 @@ __global__ __launch_bounds__(256) void k_inv_rows(const float *__restrict__ cas,
         if (rowOk) {
             float2 *o = tmp + ((size_t)a * H + ky) * N;
 #pragma unroll
-            for (int j = 0; j < R / 16; j++)
-#pragma unroll
-                for (int k1 = 0; k1 < 16; k1++) {
-                    const int px = t + 16 * j + R * k1 - off;
-                    if (px >= 0 && px < N) o[px] = v[16 * j + fft16_slot(k1)];
-                }
+            for (int m = 0; m < R; m++) {
+                const int px = t + 16 * m - off;
+                if (px >= 0 && px < N) o[px] = v[out_slot<R>(m)];
+            }
         }
     }
 }
@@ -1496,12 +1544,10 @@ __global__ __launch_bounds__(256) void k_inv_cols(const float2 *__restrict__ tmp
         __syncthreads();
         fft_m<R, 1>(v, t, ex, tw);
 #pragma unroll
-        for (int j = 0; j < R / 16; j++)
-#pragma unroll
-            for (int k1 = 0; k1 < 16; k1++) {
-                const int my = t + 16 * j + R * k1, py = my - off;
+        for (int m = 0; m < R; m++) {
+                const int my = t + 16 * m, py = my - off;
                 if (py >= 0 && py < N) {
-                    const float2 z = v[16 * j + fft16_slot(k1)];
+                    const float2 z = v[out_slot<R>(m)];
                     const int mx = c0 + 2 * f + off;
                     so[py * 33 + 2 * f] = chk(mx + my) * z.x * scale;
                     so[py * 33 + 2 * f + 1] = chk(mx + 1 + my) * z.y * scale;
@@ -1549,9 +1595,7 @@ __global__ __launch_bounds__(256) void k_fwd_cols(const float *__restrict__ imgs
         __syncthreads();
         fft_m<R, -1>(v, t, ex, tw);
 #pragma unroll
-        for (int j = 0; j < R / 16; j++)
-#pragma unroll
-            for (int k1 = 0; k1 < 16; k1++) ex[t + 16 * j + R * k1] = v[16 * j + fft16_slot(k1)];
+        for (int m = 0; m < R; m++) ex[t + 16 * m] = v[out_slot<R>(m)];
         __syncthreads();
         // C_even[ky] = (Z[ky] + conj Z[M-ky]) / 2, C_odd[ky] = (Z[ky] - conj Z[M-ky]) / (2i)
         for (int idx = tid; idx < H * 32; idx += 256) {
@@ -1597,11 +1641,9 @@ __global__ __launch_bounds__(256) void k_fwd_rows(const float2 *__restrict__ tmp
             float *o = cas + (size_t)a * M * M;
             const bool mirror = ky != 0 && ky != M / 2;
 #pragma unroll
-            for (int j = 0; j < R / 16; j++)
-#pragma unroll
-                for (int k1 = 0; k1 < 16; k1++) {
-                    const int kx = t + 16 * j + R * k1;
-                    const float2 F = v[16 * j + fft16_slot(k1)];
+            for (int m = 0; m < R; m++) {
+                    const int kx = t + 16 * m;
+                    const float2 F = v[out_slot<R>(m)];
                     const float sd = chk(kx + ky);
                     o[ky * M + kx] = (zeroNyq && (kx == 0 || ky == 0)) ? 0.f : (sd * F.x + sd * F.y) * scale;
                     if (mirror) {
@@ -2150,7 +2192,7 @@ hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hi
     return hipGetLastError();
 }
 
-bool fused_chain_supported(int M, int N) { return (M == 256 || M == 512) && N >= 1 && N <= M; }
+bool fused_chain_supported(int M, int N) { return (M == 128 || M == 256 || M == 512) && N >= 1 && N <= M; }
 
 template <int R>
 static void fused_inverse(const float *cas, float2 *tmp, float *imgs, int N, int n, float scale, hipStream_t s)
@@ -2177,7 +2219,8 @@ hipError_t launch_fused_cas_to_images(const float *cas, float2 *tmp, float *imgs
 {
     if (n <= 0) return hipSuccess;
     if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
-    if (M == 256) fused_inverse<16>(cas, tmp, imgs, N, n, scale, s);
+    if (M == 128) fused_inverse<8>(cas, tmp, imgs, N, n, scale, s);
+    else if (M == 256) fused_inverse<16>(cas, tmp, imgs, N, n, scale, s);
     else fused_inverse<32>(cas, tmp, imgs, N, n, scale, s);
     return hipGetLastError();
 }
@@ -2187,7 +2230,8 @@ hipError_t launch_fused_images_to_cas(const float *imgs, float2 *tmp, float *cas
 {
     if (n <= 0) return hipSuccess;
     if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
-    if (M == 256) fused_forward<16>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
+    if (M == 128) fused_forward<8>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
+    else if (M == 256) fused_forward<16>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
     else fused_forward<32>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
     return hipGetLastError();
 }
