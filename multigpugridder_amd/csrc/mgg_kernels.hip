@@ -411,12 +411,14 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         const long long linLo = (long long)bz0 * VV + (long long)by0 * V + bx0;
         const long long linHi = (long long)(bz0 + Dz - 1) * VV + (long long)(by0 + Dy - 1) * V + bx0 + RP - 1;
         const bool boxIn = linLo >= 0 && linHi < V3;
-        // Four planes per batch: all loads of a batch are in flight before the
-        // first LDS store waits on them.  The checked and unchecked variants
-        // are separate loops (boxIn is wave-uniform); the unchecked one
-        // addresses the volume by a 32-bit byte offset from its base
-        // (V^3 * sizeof(T) < 2^32 is required above).
-        constexpr int SB = 4;
+        // Six planes per batch: all loads of a batch are in flight before the
+        // first LDS store waits on them (2, 4 and 6 planes: 19.9, 18.5 and
+        // 18.3 ms per 10,000 directions at N=128; 8 needs 129 VGPRs, three
+        // waves per SIMD instead of four, 22.0 ms).  The checked and
+        // unchecked variants are separate loops (boxIn is wave-uniform); the
+        // unchecked one addresses the volume by a 32-bit byte offset from its
+        // base (V^3 * sizeof(T) < 2^32 is required above).
+        constexpr int SB = 6;
         const int lStep = G * VV, dStep = G * PP;
         for (int c = 0; c < nLanePass; c++) {
             const int q = ln + 64 * c;
