@@ -1484,18 +1484,22 @@ __global__ __launch_bounds__(256) void k_inv_rows(const float *__restrict__ cas,
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         const float *rowA = cas + (size_t)a * M * M + (size_t)(rowOk ? ky : 0) * M;
         const float *rowB = cas + (size_t)a * M * M + (size_t)((M - ky) & (M - 1)) * M;
+        // Unconditional loads (every address is inside the image), so that all
+        // 2R are in flight before the first use; the CAS index 0 row and column
+        // and rows beyond M/2 select zero afterwards.
+        float c1[R], c2[R];
+#pragma unroll
+        for (int n2 = 0; n2 < R; n2++) {
+            const int kx = t + 16 * n2;
+            c1[n2] = rowA[kx];
+            c2[n2] = rowB[(M - kx) & (M - 1)];
+        }
         float2 v[R];
 #pragma unroll
         for (int n2 = 0; n2 < R; n2++) {
             const int kx = t + 16 * n2;
-            float2 z = make_float2(0.f, 0.f);
-            if (rowOk && kx != 0 && ky != 0) {
-                const float c1 = rowA[kx], c2 = rowB[M - kx];
-                const float sg = chk(kx + ky);
-                z.x = sg * (0.5f * (c1 + c2));
-                z.y = sg * (0.5f * (c1 - c2));
-            }
-            v[n2] = z;
+            const float sg = (rowOk && kx != 0 && ky != 0) ? chk(kx + ky) : 0.f;
+            v[n2] = make_float2(sg * (0.5f * (c1[n2] + c2[n2])), sg * (0.5f * (c1[n2] - c2[n2])));
         }
         __syncthreads();
         fft_m<R, 1>(v, t, ex, tw);
@@ -1531,16 +1535,30 @@ __global__ __launch_bounds__(256) void k_inv_cols(const float2 *__restrict__ tmp
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         __syncthreads();
         // Z_f[ky] = R[ky][c0 + 2f] + i R[ky][c0 + 2f + 1], and by symmetry
-        // Z_f[M - ky] = conj R[ky][2f] + i conj R[ky][2f + 1].
-        for (int idx = tid; idx < H * 32; idx += 256) {
-            const int ky = idx >> 5, col = idx & 31, px = c0 + col;
-            float2 val = make_float2(0.f, 0.f);
-            if (px < N) val = tmp[((size_t)a * H + ky) * N + px];
-            const float ox = __shfl_xor(val.x, 1), oy = __shfl_xor(val.y, 1);
-            if (!(col & 1)) {
-                float2 *zf = sx + (col >> 1) * G::FFB;
-                zf[ky] = make_float2(val.x - oy, val.y + ox);
-                if (ky != 0 && ky != M / 2) zf[M - ky] = make_float2(val.x + oy, ox - val.y);
+        // Z_f[M - ky] = conj R[ky][2f] + i conj R[ky][2f + 1].  Eight loads per
+        // thread in flight before the first exchange and LDS store (lanes 2j
+        // and 2j + 1 hold neighbouring columns of one row: both in range or
+        // both not, since H * 32 is even).
+        constexpr int LB = 8;
+        for (int base = tid; base < H * 32; base += 256 * LB) {
+            float2 vals[LB];
+#pragma unroll
+            for (int q = 0; q < LB; q++) {
+                // unconditional load from a clamped (in-range) address
+                const int idx = base + 256 * q, ky = idx >> 5, px = c0 + (idx & 31);
+                const float2 raw = tmp[((size_t)a * H + min(ky, H - 1)) * N + min(px, N - 1)];
+                vals[q] = (idx < H * 32 && px < N) ? raw : make_float2(0.f, 0.f);
+            }
+#pragma unroll
+            for (int q = 0; q < LB; q++) {
+                const int idx = base + 256 * q, ky = idx >> 5, col = idx & 31;
+                const float2 val = vals[q];
+                const float ox = __shfl_xor(val.x, 1), oy = __shfl_xor(val.y, 1);
+                if (idx < H * 32 && !(col & 1)) {
+                    float2 *zf = sx + (col >> 1) * G::FFB;
+                    zf[ky] = make_float2(val.x - oy, val.y + ox);
+                    if (ky != 0 && ky != M / 2) zf[M - ky] = make_float2(val.x + oy, ox - val.y);
+                }
             }
         }
         __syncthreads();
@@ -1587,12 +1605,30 @@ __global__ __launch_bounds__(256) void k_fwd_cols(const float *__restrict__ imgs
     float *sf = reinterpret_cast<float *>(sx);
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         __syncthreads();
-        // column c0 + col goes to transform col / 2, real (even) or imaginary part
-        for (int idx = tid; idx < M * 32; idx += 256) {
-            const int my = idx >> 5, col = idx & 31, px = c0 + col, py = my - off;
-            float x = 0.f;
-            if (px < N && py >= 0 && py < N) x = chk(px + off + my) * imgs[((size_t)a * N + py) * N + px];
-            sf[((col >> 1) * G::FFB + my) * 2 + (col & 1)] = x;
+        // Column c0 + col goes to transform col / 2, real (even) or imaginary
+        // part.  The N image rows are loaded eight per thread at a time, all
+        // eight in flight before the first LDS store; the M - N padding rows
+        // are zeros.
+        constexpr int LB = 8;
+        for (int base = tid; base < N * 32; base += 256 * LB) {
+            float xv[LB];
+#pragma unroll
+            for (int q = 0; q < LB; q++) {
+                // unconditional load from a clamped (in-image) address, so that
+                // no branch separates the eight loads
+                const int idx = base + 256 * q, py = idx >> 5, col = idx & 31, px = c0 + col;
+                const float raw = imgs[((size_t)a * N + min(py, N - 1)) * N + min(px, N - 1)];
+                xv[q] = (idx < N * 32 && px < N) ? chk(px + off + py + off) * raw : 0.f;
+            }
+#pragma unroll
+            for (int q = 0; q < LB; q++) {
+                const int idx = base + 256 * q, col = idx & 31;
+                if (idx < N * 32) sf[((col >> 1) * G::FFB + (idx >> 5) + off) * 2 + (col & 1)] = xv[q];
+            }
+        }
+        for (int idx = tid; idx < (M - N) * 32; idx += 256) {
+            const int r = idx >> 5, col = idx & 31, my = r < off ? r : r + N;
+            sf[((col >> 1) * G::FFB + my) * 2 + (col & 1)] = 0.f;
         }
         __syncthreads();
         float2 v[R];
