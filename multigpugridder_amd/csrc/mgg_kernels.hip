@@ -356,7 +356,7 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     // gathers the pixels of the lanes with mine set; false (nothing done) when
     // the box does not fit the wave's LDS budget.  Uniform across the wave.
     float acc = 0.f;
-    auto stageAndGather = [&](int rlo, int rhi, bool mine) -> bool {
+    auto stageAndGather = [&](int rlo, int rhi, bool mine, auto splitT) -> bool {
         // Box from the corners: positions are affine in (i, j), so every
         // pixel's rounded coordinate lies in [floor(min corner), ceil(max
         // corner)] (the 1e-3 margin covers FP32 evaluation error), widened by
@@ -476,42 +476,84 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         // 5-tap read, so NaN and Inf reach exactly the pixels whose stencil
         // holds them, as in the reference.
         wave_lds_sync_fp();
-        if (active && mine) {
-            const int xo = X - 2 - bx0, o = xo & 3;
-            const float *bq = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (xo - o);
+        // x weights shifted by o (zeros in the unused slots) times the y weights.
+        auto shifted = [&](const float (&wx)[5], const float (&wy)[5], int o, float (&w8)[8][5]) {
             float ws[8];
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 float w = 0.f;
 #pragma unroll
                 for (int sft = 0; sft < 4; sft++)
-                    if (t - sft >= 0 && t - sft < 5) w = (o == sft) ? wi[t - sft] : w;
+                    if (t - sft >= 0 && t - sft < 5) w = (o == sft) ? wx[t - sft] : w;
                 ws[t] = w;
             }
-            float w8[8][5];
 #pragma unroll
             for (int t = 0; t < 8; t++)
 #pragma unroll
-                for (int bb = 0; bb < 5; bb++) w8[t][bb] = ws[t] * wj[bb];
-            float res = 0.f;
+                for (int bb = 0; bb < 5; bb++) w8[t][bb] = ws[t] * wy[bb];
+        };
+        // One z plane of a stencil: five rows of the 16-byte read.
+        auto planeSum = [&](const float *bq, const float (&w8)[8][5], int cc) {
+            float sc = 0.f;
 #pragma unroll
-            for (int cc = 0; cc < 5; cc++) {
-                float sc = 0.f;
-#pragma unroll
-                for (int bb = 0; bb < 5; bb++) {
-                    const float *rp = bq + cc * PP + bb * RP;
-                    const f4 q0 = *(const f4 *)rp, q1 = *(const f4 *)(rp + 4);
-                    sc = fmaf(w8[0][bb], q0.x, sc);
-                    sc = fmaf(w8[1][bb], q0.y, sc);
-                    sc = fmaf(w8[2][bb], q0.z, sc);
-                    sc = fmaf(w8[3][bb], q0.w, sc);
-                    sc = fmaf(w8[4][bb], q1.x, sc);
-                    sc = fmaf(w8[5][bb], q1.y, sc);
-                    sc = fmaf(w8[6][bb], q1.z, sc);
-                    sc = fmaf(w8[7][bb], q1.w, sc);
-                }
-                res = fmaf(wk[cc], sc, res);
+            for (int bb = 0; bb < 5; bb++) {
+                const float *rp = bq + cc * PP + bb * RP;
+                const f4 q0 = *(const f4 *)rp, q1 = *(const f4 *)(rp + 4);
+                sc = fmaf(w8[0][bb], q0.x, sc);
+                sc = fmaf(w8[1][bb], q0.y, sc);
+                sc = fmaf(w8[2][bb], q0.z, sc);
+                sc = fmaf(w8[3][bb], q0.w, sc);
+                sc = fmaf(w8[4][bb], q1.x, sc);
+                sc = fmaf(w8[5][bb], q1.y, sc);
+                sc = fmaf(w8[6][bb], q1.z, sc);
+                sc = fmaf(w8[7][bb], q1.w, sc);
             }
+            return sc;
+        };
+        float res = 0.f;
+        bool gathered = false;
+        if constexpr (!decltype(splitT)::value) {
+            if (active && mine) {
+                const int xo = X - 2 - bx0, o = xo & 3;
+                const float *bq = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (xo - o);
+                float w8[8][5];
+                shifted(wi, wj, o, w8);
+#pragma unroll
+                for (int cc = 0; cc < 5; cc++) res = fmaf(wk[cc], planeSum(bq, w8, cc), res);
+                gathered = true;
+            }
+        } else {
+            // Half tile: the lanes of the other half (lane ^ 32; their own
+            // pixels are not in this box) gather planes 3 and 4 of their
+            // partner's stencil while the owner gathers planes 0-2, so the wave
+            // reads 15 rows per pixel instead of 25; the owner adds the parts.
+            const int pX = __shfl_xor(X, 32), pY = __shfl_xor(Y, 32), pZ = __shfl_xor(Z, 32);
+            const int pAct = __shfl_xor(active ? 1 : 0, 32);
+            float gwi[5], gwj[5];
+#pragma unroll
+            for (int t = 0; t < 5; t++) {
+                const float a5 = __shfl_xor(wi[t], 32), b5 = __shfl_xor(wj[t], 32);
+                gwi[t] = mine ? wi[t] : a5;
+                gwj[t] = mine ? wj[t] : b5;
+            }
+            const float pk3 = __shfl_xor(wk[3], 32), pk4 = __shfl_xor(wk[4], 32);
+            const int gX = mine ? X : pX, gY = mine ? Y : pY, gZ = mine ? Z : pZ + 3;
+            const float k0 = mine ? wk[0] : pk3, k1 = mine ? wk[1] : pk4;
+            float part = 0.f;
+            if (mine ? active : pAct != 0) {
+                const int xo = gX - 2 - bx0, o = xo & 3;
+                const float *bq = box + (gZ - 2 - bz0) * PP + (gY - 2 - by0) * RP + (xo - o);
+                float w8[8][5];
+                shifted(gwi, gwj, o, w8);
+                part = fmaf(k0, planeSum(bq, w8, 0), part);
+                part = fmaf(k1, planeSum(bq, w8, 1), part);
+                if (mine) part = fmaf(wk[2], planeSum(bq, w8, 2), part);
+            }
+            const float other = __shfl_xor(part, 32);
+            res = part + other;
+            gathered = active && mine;
+        }
+        if (gathered) {
             if (!__builtin_isfinite(res)) {
                 const float *bp = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (X - 2 - bx0);
                 res = 0.f;
@@ -536,11 +578,11 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     // The whole 8x8 tile; if its box is too large (27% of the tiles for the
     // grid directions at N=128), its two 8x4 halves one after the other; a
     // half that still does not fit gathers from global memory.
-    bool done = stageAndGather(0, FPT - 1, true);
+    bool done = stageAndGather(0, FPT - 1, true, std::false_type{});
     if (!done) {
         const bool top = (ln >> 3) < FPT / 2;
-        const bool d0 = stageAndGather(0, FPT / 2 - 1, top);
-        const bool d1 = stageAndGather(FPT / 2, FPT - 1, !top);
+        const bool d0 = stageAndGather(0, FPT / 2 - 1, top, std::true_type{});
+        const bool d1 = stageAndGather(FPT / 2, FPT - 1, !top, std::true_type{});
         done = top ? d0 : d1;
     }
     if (!done && active) {
