@@ -239,7 +239,7 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
 constexpr int FPT = 8;          // tile edge (pixels)
 // Per-wave staged box budget (floats).  Four boxes take 40,960 bytes of LDS,
 // a quarter of the CU's 160 KB, so four workgroups share a CU (the kernel's
-// 113 VGPRs allow four waves per SIMD); 73% of the 8x8 tiles' boxes fit whole
+// 121 VGPRs allow four waves per SIMD, at most 128 do); 73% of the 8x8 tiles' boxes fit whole
 // on the grid and random workloads at N=128 (median 2,400 floats), the rest
 // are staged as two 8x4 halves.  Occupancy matters more than the halves'
 // extra work: three workgroups per CU with a 3,392-float budget (nearly every
