@@ -308,7 +308,7 @@ def main():
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--ntheta", type=int, default=100)
     ap.add_argument("--nphi", type=int, default=100)
-    ap.add_argument("--chunk", type=int, default=1000, help="directions per launch")
+    ap.add_argument("--chunk", type=int, default=2000, help="directions per launch")
     ap.add_argument("--streams", type=int, default=2,
                     help="streams the chunks of a step are spread over (image chains overlap projectors)")
     ap.add_argument("--cpu-sample", type=int, default=2000,
