@@ -36,6 +36,34 @@ def test_forward_project_object(cuda, oracle, ker32, run_fft_on_gpu, n, p, strea
     g.delete()
 
 
+def test_chunk_beyond_grid_limit(cuda, oracle, ker32):
+    """One chunk of more images than a grid dimension holds (65,535): with one
+    stream and room for every direction, the gridder runs all 70,000
+    directions (N=64, the fused M=128 image chains) in one launch each, so the
+    chain kernels stride over images.  Forward: images on both sides of the
+    limit against the oracle.  Back: all images zero but six, so by linearity
+    the oracle's back projection of those six is the exact reference."""
+    from multigpugridder_amd import MultiGPUGridder, blob_volume, random_directions
+
+    n, p = 64, 70000
+    m, v = 2 * n, 2 * n + 6
+    idx = [0, 1, 65534, 65535, 65536, p - 1]
+    print(f"chunk beyond grid limit n={n} p={p}", flush=True)
+    vol = blob_volume(n, 9)
+    axes = random_directions(p, 23)
+    g = MultiGPUGridder(n, p, 2, nStreamsFP=1, nStreamsBP=1, MaxAxesToAllocate=p)
+    g.setVolume(vol)
+    imgs = g.forwardProject(axes)
+    _, ref = _oracle_forward(oracle, ker32, vol, axes[idx], n)
+    assert rel_l2(imgs[idx], ref) < TOL
+    sparse = np.zeros((p, n, n), dtype=np.float32)
+    sparse[idx] = np.random.default_rng(3).standard_normal((len(idx), n, n)).astype(np.float32)
+    g.backProject(sparse, axes)
+    cas_imgs = oracle.imgs_to_cas_imgs(sparse[idx], m).astype(np.float32)
+    assert rel_l2(g.casVolumeToHost(), oracle.back_project(cas_imgs, axes[idx], ker32, v)) < TOL
+    g.delete()
+
+
 @pytest.mark.parametrize("run_fft_on_gpu", [1, 0])
 def test_back_project_object(cuda, oracle, ker32, run_fft_on_gpu):
     from multigpugridder_amd import MultiGPUGridder, random_directions
