@@ -40,8 +40,11 @@ def _setup(n, p, seed, oracle, grid=False):
 # ---------------------------------------------------------------------------
 # Forward projection
 # ---------------------------------------------------------------------------
+# M = 60 and 100 are not multiples of the 8x8 tile: the last tile row and
+# column hang over the image edge.
 @pytest.mark.parametrize("n,p,seed,grid", [(8, 3, 1, False), (16, 8, 1, False), (32, 40, 2, False),
-                                           (32, 0, 3, (4, 10)), (64, 64, 4, False)])
+                                           (32, 0, 3, (4, 10)), (64, 64, 4, False), (30, 12, 5, False),
+                                           (50, 20, 6, False)])
 def test_forward_project_f32(cuda, oracle, ker32, n, p, seed, grid):
     from multigpugridder_amd import ops
 
