@@ -1271,13 +1271,14 @@ __global__ void k_herm2d_to_cas(const float2 *__restrict__ herm, float *__restri
 }
 
 // ---------------------------------------------------------------------------
-// Fused image chains for M = 128, 256 and 512.  The general chains above make
+// Fused image chains for M = 128, 192, 256, 384 and 512.  The general chains above make
 // five or six HBM passes per image (CAS -> Hermitian half spectrum, two rocFFT
 // passes, crop; or the reverse).  These make two each, with the M-point FFT
 // done in LDS as four-step 16 x R (M = 16 R; 16 threads per transform, the
 // R-point and 16-point DFTs in registers, one twiddled exchange through LDS;
 // the 32-point DFT is two 16-point DFTs and a radix-2 step, the 8-point DFT
-// two 4-point DFTs and a radix-2 step), and are ordered
+// two 4-point DFTs and a radix-2 step, the 12- and 24-point DFTs 4- and
+// 8-point DFTs and a radix-3 step), and are ordered
 // so that CAS images are read and written as whole rows:
 //   CAS images -> images: (1) k_inv_rows: per spectrum row ky <= M/2 the full
 //   complex row is built from CAS rows ky and M - ky as they are read,
@@ -1415,11 +1416,96 @@ __device__ __forceinline__ void fft8(float2 (&v)[8])
     v[7] = make_float2(e3.x - o3.x, e3.y - o3.y);
 }
 
+// cos, sin of 2 pi j / 24, j = 0..14
+__device__ __constant__ const F16Tw kTw24[15] = {
+    {1.00000000000000000f, 0.00000000000000000f},
+    {0.96592582628906831f, 0.25881904510252074f},
+    {0.86602540378443871f, 0.49999999999999994f},
+    {0.70710678118654757f, 0.70710678118654746f},
+    {0.50000000000000011f, 0.86602540378443860f},
+    {0.25881904510252074f, 0.96592582628906831f},
+    {0.00000000000000006f, 1.00000000000000000f},
+    {-0.25881904510252063f, 0.96592582628906831f},
+    {-0.49999999999999978f, 0.86602540378443871f},
+    {-0.70710678118654746f, 0.70710678118654757f},
+    {-0.86602540378443871f, 0.49999999999999994f},
+    {-0.96592582628906820f, 0.25881904510252102f},
+    {-1.00000000000000000f, 0.00000000000000012f},
+    {-0.96592582628906831f, -0.25881904510252079f},
+    {-0.86602540378443882f, -0.49999999999999972f}};
+
+// 3-point DFT in place (natural order).
+template <int S>
+__device__ __forceinline__ void fft3(float2 &a, float2 &b, float2 &c)
+{
+    constexpr float s3 = 0.86602540378443865f;  // sin(2 pi / 3)
+    const float2 t1 = make_float2(b.x + c.x, b.y + c.y);
+    const float2 t2 = make_float2(a.x - 0.5f * t1.x, a.y - 0.5f * t1.y);
+    const float2 d = make_float2(b.x - c.x, b.y - c.y);
+    const float2 r = S > 0 ? make_float2(-s3 * d.y, s3 * d.x) : make_float2(s3 * d.y, -s3 * d.x);
+    a = make_float2(a.x + t1.x, a.y + t1.y);
+    b = make_float2(t2.x + r.x, t2.y + r.y);
+    c = make_float2(t2.x - r.x, t2.y - r.y);
+}
+
+// 12- and 24-point DFTs in registers, natural order in and out: n = 3 p + q,
+// k = k1 + (R / 3) k2; a (R / 3)-point DFT over p for each q, twiddles
+// w_R^(q k1), then a 3-point DFT over q for each k1.
+template <int S>
+__device__ __forceinline__ void fft12(float2 (&v)[12])
+{
+    float2 b[3][4];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+        float2 a0 = v[q], a1 = v[3 + q], a2 = v[6 + q], a3 = v[9 + q];
+        fft4<S>(a0, a1, a2, a3);
+        b[q][0] = a0; b[q][1] = a1; b[q][2] = a2; b[q][3] = a3;
+#pragma unroll
+        for (int k1 = 1; k1 < 4; k1++) {
+            if (q == 0) continue;
+            const F16Tw w = kTw24[2 * q * k1];
+            b[q][k1] = cmulw(b[q][k1], w.c, S > 0 ? w.s : -w.s);
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; k1++) {
+        float2 x0 = b[0][k1], x1 = b[1][k1], x2 = b[2][k1];
+        fft3<S>(x0, x1, x2);
+        v[k1] = x0; v[k1 + 4] = x1; v[k1 + 8] = x2;
+    }
+}
+
+template <int S>
+__device__ __forceinline__ void fft24(float2 (&v)[24])
+{
+    float2 b[3][8];
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+#pragma unroll
+        for (int p8 = 0; p8 < 8; p8++) b[q][p8] = v[3 * p8 + q];
+        fft8<S>(b[q]);
+#pragma unroll
+        for (int k1 = 1; k1 < 8; k1++) {
+            if (q == 0) continue;
+            const F16Tw w = kTw24[q * k1];
+            b[q][k1] = cmulw(b[q][k1], w.c, S > 0 ? w.s : -w.s);
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < 8; k1++) {
+        float2 x0 = b[0][k1], x1 = b[1][k1], x2 = b[2][k1];
+        fft3<S>(x0, x1, x2);
+        v[k1] = x0; v[k1 + 8] = x1; v[k1 + 16] = x2;
+    }
+}
+
 template <int R, int S>
 __device__ __forceinline__ void fft_regs(float2 (&v)[R])
 {
     if constexpr (R == 8) fft8<S>(v);
+    else if constexpr (R == 12) fft12<S>(v);
     else if constexpr (R == 16) fft16<S>(v);
+    else if constexpr (R == 24) fft24<S>(v);
     else fft32<S>(v);
 }
 
@@ -1427,14 +1513,36 @@ __device__ __forceinline__ void fft_regs(float2 (&v)[R])
 template <int R>
 __device__ __forceinline__ constexpr int fftR_slot(int k)
 {
-    return R == 8 ? k : (R == 16 ? fft16_slot(k) : (k < 16 ? fft16_slot(k) : 16 + fft16_slot(k - 16)));
+    return (R == 8 || R == 12 || R == 24) ? k
+         : (R == 16 ? fft16_slot(k) : (k < 16 ? fft16_slot(k) : 16 + fft16_slot(k - 16)));
 }
 
-// Register of X[t + 16 m] (m < R) after fft_m<R> (see there).
+// After fft_m<R>, thread t holds X[out_idx<R>(t, m)] in v[out_slot<R>(m)],
+// m < R (see there).
 template <int R>
 __device__ __forceinline__ constexpr int out_slot(int m)
 {
-    return R == 8 ? m : (R == 16 ? fft16_slot(m) : 16 * (m & 1) + fft16_slot(m >> 1));
+    return (R == 8 || R == 12 || R == 24) ? m : (R == 16 ? fft16_slot(m) : 16 * (m & 1) + fft16_slot(m >> 1));
+}
+
+template <int R>
+__device__ __forceinline__ int out_idx(int t, int m)
+{
+    if constexpr (R == 12 || R == 24) {
+        constexpr int D = R == 12 ? 4 : 2, L = 16 / D;
+        const int P = t + 16 * (m / L), mm = m % L;
+        return P % R + R * (D * mm + P / R);
+    } else {
+        return t + 16 * m;
+    }
+}
+
+// Index of the mirrored row or column, (M - k) mod M, for 0 <= k < M.
+template <int M>
+__device__ __forceinline__ int mirror_idx(int k)
+{
+    if constexpr ((M & (M - 1)) == 0) return (M - k) & (M - 1);
+    else return k == 0 ? 0 : M - k;
 }
 
 constexpr int FFP = 17;  // exchange pitch (float2)
@@ -1449,12 +1557,16 @@ template <int R> struct FftGeom {
 
 // M-point DFT (M = 16 R) by the 16 threads t = 0..15 of one transform,
 // four-step 16 x R: R-point DFTs in registers, twiddles, exchange through
-// LDS, 16-point DFTs.  On entry v[n2] holds x[t + 16 n2] (n2 < R); on exit
-// v[out_slot<R>(m)] holds X[t + 16 m] (m < R).  With R >= 16 thread t does the
-// 16-point DFTs of k2 = t + 16 j (outputs X[k2 + R k1]); with R = 8 each of
-// the eight 16-point DFTs is shared by threads k2 and k2 + 8 through one
-// radix-2 decimation in frequency (thread h = t / 8 takes the outputs
-// k1 = 2 m + h, an 8-point DFT of (z[n] + (-1)^h z[n + 8]) w16^(n h)).
+// LDS, 16-point DFTs (outputs X[k2 + R k1], k2 < R, k1 < 16).  On entry v[n2]
+// holds x[t + 16 n2] (n2 < R); on exit v[out_slot<R>(m)] holds
+// X[out_idx<R>(t, m)] (m < R).  With R = 16, 32 thread t does the 16-point
+// DFTs of k2 = t + 16 j (out_idx = t + 16 m).  Otherwise each 16-point DFT is
+// shared by D threads through one radix-D decimation in frequency: the share
+// h < D takes the outputs k1 = D mm + h, an L = 16 / D point DFT over n of
+// w16^(n h) sum_e z[n + L e] w_D^(e h).  R = 8: D = 2, thread t takes
+// k2 = t % 8, h = t / 8 (out_idx = t + 16 m again); R = 12 (D = 4) and
+// R = 24 (D = 2): thread t takes the three (k2, h) = (P % R, P / R),
+// P = t + 16 j, j < 3.
 // ex is the transform's FFB-float2 LDS exchange area, tw the M twiddles
 // e^{2 pi i q / M}.  Contains workgroup barriers: every thread of the block
 // must call it.
@@ -1479,6 +1591,39 @@ __device__ __forceinline__ void fft_m(float2 (&v)[R], int t, float2 *ex, const f
             v[n] = cmulw(d, w.x, S > 0 ? w.y : -w.y);
         }
         fft8<S>(v);
+    } else if constexpr (R == 12 || R == 24) {
+        constexpr int D = R == 12 ? 4 : 2, L = 16 / D;
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            const int P = t + 16 * j, k2 = P % R, h = P / R;
+            float2 b[L];
+#pragma unroll
+            for (int n = 0; n < L; n++) {
+                float2 sm;
+                if constexpr (D == 2) {
+                    const float2 z0 = ex[k2 * FFP + n], z1 = ex[k2 * FFP + n + 8];
+                    const float sg = h ? -1.f : 1.f;
+                    sm = make_float2(fmaf(sg, z1.x, z0.x), fmaf(sg, z1.y, z0.y));
+                } else {
+                    sm = ex[k2 * FFP + n];
+#pragma unroll
+                    for (int e = 1; e < 4; e++) {
+                        // z * (S i)^(e h): quarter turns, counter-clockwise for S > 0
+                        const float2 z = ex[k2 * FFP + n + 4 * e];
+                        const int ph = (S > 0 ? e * h : 4 * e - e * h) & 3;
+                        const float2 r = ph == 0 ? z : ph == 1 ? make_float2(-z.y, z.x)
+                                       : ph == 2 ? make_float2(-z.x, -z.y) : make_float2(z.y, -z.x);
+                        sm = make_float2(sm.x + r.x, sm.y + r.y);
+                    }
+                }
+                const float2 w = tw[R * n * h];  // w16^(n h)
+                b[n] = cmulw(sm, w.x, S > 0 ? w.y : -w.y);
+            }
+            if constexpr (L == 4) fft4<S>(b[0], b[1], b[2], b[3]);
+            else fft8<S>(b);
+#pragma unroll
+            for (int mm = 0; mm < L; mm++) v[j * L + mm] = b[mm];
+        }
     } else {
 #pragma unroll
         for (int j = 0; j < R / 16; j++) {
@@ -1525,7 +1670,7 @@ __global__ __launch_bounds__(256) void k_inv_rows(const float *__restrict__ cas,
     float2 *ex = sx + f * G::FFB;
     for (int a = blockIdx.y; a < n; a += gridDim.y) {
         const float *rowA = cas + (size_t)a * M * M + (size_t)(rowOk ? ky : 0) * M;
-        const float *rowB = cas + (size_t)a * M * M + (size_t)((M - ky) & (M - 1)) * M;
+        const float *rowB = cas + (size_t)a * M * M + (size_t)mirror_idx<M>(ky) * M;
         // Unconditional loads (every address is inside the image), so that all
         // 2R are in flight before the first use; the CAS index 0 row and column
         // and rows beyond M/2 select zero afterwards.
@@ -1534,7 +1679,7 @@ __global__ __launch_bounds__(256) void k_inv_rows(const float *__restrict__ cas,
         for (int n2 = 0; n2 < R; n2++) {
             const int kx = t + 16 * n2;
             c1[n2] = rowA[kx];
-            c2[n2] = rowB[(M - kx) & (M - 1)];
+            c2[n2] = rowB[mirror_idx<M>(kx)];
         }
         float2 v[R];
 #pragma unroll
@@ -1549,7 +1694,7 @@ __global__ __launch_bounds__(256) void k_inv_rows(const float *__restrict__ cas,
             float2 *o = tmp + ((size_t)a * H + ky) * N;
 #pragma unroll
             for (int m = 0; m < R; m++) {
-                const int px = t + 16 * m - off;
+                const int px = out_idx<R>(t, m) - off;
                 if (px >= 0 && px < N) o[px] = v[out_slot<R>(m)];
             }
         }
@@ -1611,7 +1756,7 @@ __global__ __launch_bounds__(256) void k_inv_cols(const float2 *__restrict__ tmp
         fft_m<R, 1>(v, t, ex, tw);
 #pragma unroll
         for (int m = 0; m < R; m++) {
-                const int my = t + 16 * m, py = my - off;
+                const int my = out_idx<R>(t, m), py = my - off;
                 if (py >= 0 && py < N) {
                     const float2 z = v[out_slot<R>(m)];
                     const int mx = c0 + 2 * f + off;
@@ -1679,14 +1824,14 @@ __global__ __launch_bounds__(256) void k_fwd_cols(const float *__restrict__ imgs
         __syncthreads();
         fft_m<R, -1>(v, t, ex, tw);
 #pragma unroll
-        for (int m = 0; m < R; m++) ex[t + 16 * m] = v[out_slot<R>(m)];
+        for (int m = 0; m < R; m++) ex[out_idx<R>(t, m)] = v[out_slot<R>(m)];
         __syncthreads();
         // C_even[ky] = (Z[ky] + conj Z[M-ky]) / 2, C_odd[ky] = (Z[ky] - conj Z[M-ky]) / (2i)
         for (int idx = tid; idx < H * 32; idx += 256) {
             const int ky = idx >> 5, col = idx & 31, px = c0 + col;
             if (px >= N) continue;
             const float2 *zf = sx + (col >> 1) * G::FFB;
-            const float2 z = zf[ky], zm = zf[(M - ky) & (M - 1)];
+            const float2 z = zf[ky], zm = zf[mirror_idx<M>(ky)];
             tmp[((size_t)a * H + ky) * N + px] = (col & 1)
                 ? make_float2(0.5f * (z.y + zm.y), -0.5f * (z.x - zm.x))
                 : make_float2(0.5f * (z.x + zm.x), 0.5f * (z.y - zm.y));
@@ -1726,12 +1871,12 @@ __global__ __launch_bounds__(256) void k_fwd_rows(const float2 *__restrict__ tmp
             const bool mirror = ky != 0 && ky != M / 2;
 #pragma unroll
             for (int m = 0; m < R; m++) {
-                    const int kx = t + 16 * m;
+                    const int kx = out_idx<R>(t, m);
                     const float2 F = v[out_slot<R>(m)];
                     const float sd = chk(kx + ky);
                     o[ky * M + kx] = (zeroNyq && (kx == 0 || ky == 0)) ? 0.f : (sd * F.x + sd * F.y) * scale;
                     if (mirror) {
-                        const int x = (M - kx) & (M - 1);  // CAS[M - ky][x] from conj F[ky][(M - x) % M]
+                        const int x = mirror_idx<M>(kx);  // CAS[M - ky][x] from conj F[ky][(M - x) % M]
                         const float sm = chk(x + ky);
                         o[(M - ky) * M + x] = (zeroNyq && x == 0) ? 0.f : (sm * F.x - sm * F.y) * scale;
                     }
@@ -2276,7 +2421,10 @@ hipError_t launch_herm2d_to_cas(const float2 *herm, float *cas, int M, int n, hi
     return hipGetLastError();
 }
 
-bool fused_chain_supported(int M, int N) { return (M == 128 || M == 256 || M == 512) && N >= 1 && N <= M; }
+bool fused_chain_supported(int M, int N)
+{
+    return (M == 128 || M == 192 || M == 256 || M == 384 || M == 512) && N >= 1 && N <= M;
+}
 
 template <int R>
 static void fused_inverse(const float *cas, float2 *tmp, float *imgs, int N, int n, float scale, hipStream_t s)
@@ -2303,9 +2451,13 @@ hipError_t launch_fused_cas_to_images(const float *cas, float2 *tmp, float *imgs
 {
     if (n <= 0) return hipSuccess;
     if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
-    if (M == 128) fused_inverse<8>(cas, tmp, imgs, N, n, scale, s);
-    else if (M == 256) fused_inverse<16>(cas, tmp, imgs, N, n, scale, s);
-    else fused_inverse<32>(cas, tmp, imgs, N, n, scale, s);
+    switch (M) {
+    case 128: fused_inverse<8>(cas, tmp, imgs, N, n, scale, s); break;
+    case 192: fused_inverse<12>(cas, tmp, imgs, N, n, scale, s); break;
+    case 256: fused_inverse<16>(cas, tmp, imgs, N, n, scale, s); break;
+    case 384: fused_inverse<24>(cas, tmp, imgs, N, n, scale, s); break;
+    default: fused_inverse<32>(cas, tmp, imgs, N, n, scale, s); break;
+    }
     return hipGetLastError();
 }
 
@@ -2314,9 +2466,13 @@ hipError_t launch_fused_images_to_cas(const float *imgs, float2 *tmp, float *cas
 {
     if (n <= 0) return hipSuccess;
     if (!fused_chain_supported(M, N)) return hipErrorInvalidValue;
-    if (M == 128) fused_forward<8>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
-    else if (M == 256) fused_forward<16>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
-    else fused_forward<32>(imgs, tmp, cas, N, n, scale, zeroNyq, s);
+    switch (M) {
+    case 128: fused_forward<8>(imgs, tmp, cas, N, n, scale, zeroNyq, s); break;
+    case 192: fused_forward<12>(imgs, tmp, cas, N, n, scale, zeroNyq, s); break;
+    case 256: fused_forward<16>(imgs, tmp, cas, N, n, scale, zeroNyq, s); break;
+    case 384: fused_forward<24>(imgs, tmp, cas, N, n, scale, zeroNyq, s); break;
+    default: fused_forward<32>(imgs, tmp, cas, N, n, scale, zeroNyq, s); break;
+    }
     return hipGetLastError();
 }
 

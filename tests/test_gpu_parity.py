@@ -274,12 +274,14 @@ def test_back_project_adjoint(cuda, oracle, ker32, n, p):
 # ---------------------------------------------------------------------------
 # Image and volume chains (rocFFT + fused pre/post kernels)
 # ---------------------------------------------------------------------------
-# m = 128, 256 and 512 run the fused two-pass chains (any n <= m with m - n even, an
-# odd offset (m - n) / 2 included); other m run the rocFFT chains (300 images: full
-# batches of 256 and a remainder split into power-of-two batches).
+# m = 128, 192, 256, 384 and 512 run the fused two-pass chains (any n <= m with
+# m - n even, an odd offset (m - n) / 2 included); other m run the rocFFT chains
+# (300 images: full batches of 256 and a remainder split into power-of-two batches).
 @pytest.mark.parametrize("n,m,p", [(8, 16, 3), (16, 32, 7), (100, 200, 3), (50, 100, 300),
                                    (64, 128, 5), (2, 128, 3), (30, 128, 7), (128, 128, 2), (100, 128, 260),
+                                   (96, 192, 5), (2, 192, 3), (50, 192, 4), (192, 192, 2), (100, 192, 260),
                                    (128, 256, 300), (100, 256, 5), (6, 256, 3), (254, 256, 2), (256, 256, 2),
+                                   (192, 384, 3), (98, 384, 2), (384, 384, 2),
                                    (256, 512, 3), (100, 512, 2)])
 def test_image_chains(cuda, oracle, n, m, p):
     from multigpugridder_amd import ops
@@ -295,10 +297,11 @@ def test_image_chains(cuda, oracle, n, m, p):
 
 
 @pytest.mark.parametrize("n,m,p", [(128, 256, 40), (98, 256, 3), (256, 512, 3), (64, 128, 9), (98, 128, 3),
-                                   (24, 48, 5)])
+                                   (96, 192, 7), (192, 384, 3), (24, 48, 5)])
 def test_image_chain_transposes(cuda, oracle, n, m, p):
     """The differentiable projector's transposes of the image chains (scale and
-    index-0 zeroing), on the fused (m = 128, 256, 512) and the rocFFT chains."""
+    index-0 zeroing), on the fused (m = 128, 192, 256, 384, 512) and the rocFFT
+    chains."""
     from multigpugridder_amd import ops
 
     print(f"image chain transposes n={n} m={m} p={p}", flush=True)
