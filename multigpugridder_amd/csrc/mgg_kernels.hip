@@ -1,13 +1,18 @@
 // HIP kernels for the Fourier-domain projection gridder, written for gfx950
 // (CDNA4, wave64).
 //
-// Numerics contract.  This file is compiled with -ffp-contract=off: every
-// FP32 expression that feeds a discrete decision (rotated coordinates, roundf
-// of them, kernel-table indices, the mask and slab tests) is evaluated with
-// separately rounded operations in the order the reference kernels write
-// them, so rounding decisions match the CPU oracle bit for bit.  Fused
-// multiply-adds appear only where written explicitly (fmaf), and only in
-// accumulations.
+// Numerics contract.  This file is compiled with -ffp-contract=off, so every
+// fused multiply-add is written explicitly (fmaf).  The FP32 expressions that
+// feed discrete decisions (rotated coordinates, roundf of them, kernel-table
+// indices, the mask and slab tests) follow the reference kernels as nvcc
+// compiles them by default (--fmad=true; the reference's build passes no
+// flags): a multiply feeding an add is contracted, without reassociation,
+// each add of two products fusing its first operand:
+//   a*x + b*y + c           -> fl(fma(a, x, fl(b*y)) + c)       (ref_sum2c)
+//   x0*a0 + x1*a1 + x2*a2   -> fma(x2, a2, fma(x0, a0, fl(x1*a1)))  (ref_sum3)
+//   r*kerScale + kerCenter  -> fma(r, kerScale, kerCenter)
+// The CPU oracle restates the same reading, so rounding decisions match it
+// bit for bit; accumulation order is free (FP32 here, FP64 in the oracle).
 //
 // Layout: column-major, x fastest.  vol[x + y*V + z*V*V];
 // CAS image a: img[a*M*M + i + j*M]; a direction is 9 floats [u; v; n].
@@ -26,15 +31,29 @@ typedef unsigned int u3 __attribute__((ext_vector_type(3)));
 constexpr int kThreads = 256;
 
 // ---------------------------------------------------------------------------
-// Kernel-table index (gpuForwardProject.cu:63-67, gpuBackProject.cu:79-82):
-// clamp to [-convW, convW], scale, round half away from zero, clamp to table.
+// Decision arithmetic as contracted by nvcc (see the numerics contract above).
 // ---------------------------------------------------------------------------
+// a*x + b*y + c: a pixel's rotated coordinate (gpuForwardProject.cu:53-55) or
+// a back-projection tap position (gpuBackProject.cu:77,84,91).
+__device__ __forceinline__ float ref_sum2c(float a, float x, float b, float y, float c)
+{
+    return fmaf(a, x, b * y) + c;
+}
+// x0*a0 + x1*a1 + x2*a2: a voxel's image-frame coordinate before its constant
+// (gpuBackProject.cu:62-64).
+__device__ __forceinline__ float ref_sum3(float x0, float a0, float x1, float a1, float x2, float a2)
+{
+    return fmaf(x2, a2, fmaf(x0, a0, x1 * a1));
+}
+
+// Kernel-table index (gpuForwardProject.cu:63-67, gpuBackProject.cu:79-82):
+// clamp to [-convW, convW], scale and offset (one FMA), round half away from
+// zero, clamp to table.
 __device__ __forceinline__ int kb_index(float r, float fW, float kerScale, float kerCenter,
                                         int K)
 {
     r = fminf(fmaxf(r, -fW), fW);
-    float s = r * kerScale;
-    s = s + kerCenter;
+    const float s = fmaf(r, kerScale, kerCenter);
     int idx = (int)roundf(s);
     idx = min(max(idx, 0), K - 1);
     return idx;
@@ -151,9 +170,9 @@ __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
         const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
         const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
         const float fi = (float)(i - ci), fj = (float)(j - ci);
-        const float x = u0 * fi + v0 * fj + (float)cv;
-        const float y = u1 * fi + v1 * fj + (float)cv;
-        const float z = u2 * fi + v2 * fj + (float)cv;
+        const float x = ref_sum2c(u0, fi, v0, fj, (float)cv);
+        const float y = ref_sum2c(u1, fi, v1, fj, (float)cv);
+        const float z = ref_sum2c(u2, fi, v2, fj, (float)cv);
         const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
         const T *vol = (const T *)p.vol;
         const long long V3 = (long long)V * V * V;
@@ -270,8 +289,7 @@ __device__ __forceinline__ int cvt_rpi(float x)
 // keeps any index, even from a non-finite coordinate, inside the table.
 __device__ __forceinline__ float kb_fast_fp(const float *__restrict__ ker, float r)
 {
-    float s = r * 125.0f;
-    s = s + 250.0f;
+    const float s = fmaf(r, 125.0f, 250.0f);
     const int i = cvt_rpi(s);
     return *(const float *)((const char *)ker + (unsigned)min(max(i, 0), 500) * 4u);
 }
@@ -330,9 +348,9 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
     const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
     const float fi = (float)(i - ci), fj = (float)(j - ci);
-    const float x = u0 * fi + v0 * fj + (float)cv;
-    const float y = u1 * fi + v1 * fj + (float)cv;
-    const float z = u2 * fi + v2 * fj + (float)cv;
+    const float x = ref_sum2c(u0, fi, v0, fj, (float)cv);
+    const float y = ref_sum2c(u1, fi, v1, fj, (float)cv);
+    const float z = ref_sum2c(u2, fi, v2, fj, (float)cv);
     const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
 
     const long long V3 = (long long)V * V * V;
@@ -647,9 +665,9 @@ __global__ __launch_bounds__(kThreads) void adj_kernel(FPArgs p, const float *__
     const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
     const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
     const float fi = (float)(i - ci), fj = (float)(j - ci);
-    const float x = u0 * fi + v0 * fj + (float)cv;
-    const float y = u1 * fi + v1 * fj + (float)cv;
-    const float z = u2 * fi + v2 * fj + (float)cv;
+    const float x = ref_sum2c(u0, fi, v0, fj, (float)cv);
+    const float y = ref_sum2c(u1, fi, v1, fj, (float)cv);
+    const float z = ref_sum2c(u2, fi, v2, fj, (float)cv);
     const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
     const long long V3 = (long long)V * V * V;
     const int W = p.convW;
@@ -685,6 +703,7 @@ struct BPArgs {
     int convW, bricksX, bricksXY;
     int maskD2;  // largest integer d2 with sqrtf((float)d2) <= maskR (-1: none)
     int posRound;  // bp_fast_kernel: in-mask voxels of orthonormal frames project to fi, fj >= 0.5
+    int maskLo;    // smallest coordinate of an in-mask voxel (V/2 - floor(sqrt(maskD2)))
     const int *order;  // bp_fast_kernel: brick id per work item, heaviest first
     unsigned perXcd, totalWork;
 };
@@ -765,32 +784,34 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BPArgs p)
         for (int q = 0; q < total; q++) {
             const float *ax = sax + q * 9;
             const float n0 = ax[6], n1 = ax[7], n2 = ax[8];
-            const float fk = dvi * n0 + dvj * n1 + dvk * n2;
+            const float fk = ref_sum3(dvi, n0, dvj, n1, dvk, n2);
             if (!(active && fabsf(fk) <= p.hw)) continue;
             const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
             const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
-            const float fi = dvi * u0 + dvj * u1 + dvk * u2 + (float)ci;
-            const float fj = dvi * v0 + dvj * v1 + dvk * v2 + (float)ci;
+            const float fi = ref_sum3(dvi, u0, dvj, u1, dvk, u2) + (float)ci;
+            const float fj = ref_sum3(dvi, v0, dvj, v1, dvk, v2) + (float)ci;
             const int I = (int)roundf(fi), J = (int)roundf(fj);
             const float *im = p.img + (size_t)sidx[q] * M2;
             float cum = 0.f, dsum = 0.f;
             if (p.convW == 2) {
-                float Ax[5], Ay[5], Az[5], Bx[5], By[5], Bz[5];
+                // Tap position fl(fma(di, u, fl(dj*v)) + c): the products
+                // dj*v are rounded once per row, di stays exact.
+                float Di[5], Bx[5], By[5], Bz[5];
 #pragma unroll
                 for (int s = 0; s < 5; s++) {
-                    const float di = (float)(I - 2 + s - ci), dj = (float)(J - 2 + s - ci);
-                    Ax[s] = di * u0; Ay[s] = di * u1; Az[s] = di * u2;
+                    const float dj = (float)(J - 2 + s - ci);
+                    Di[s] = (float)(I - 2 + s - ci);
                     Bx[s] = dj * v0; By[s] = dj * v1; Bz[s] = dj * v2;
                 }
 #pragma unroll
                 for (int jj = 0; jj < 5; jj++) {
 #pragma unroll
                     for (int ii = 0; ii < 5; ii++) {
-                        const float px = Ax[ii] + Bx[jj] + cvf;
+                        const float px = fmaf(Di[ii], u0, Bx[jj]) + cvf;
                         float w = sker[kb_index(px - fvi, p.fW, p.kerScale, p.kerCenter, p.K)];
-                        const float py = Ay[ii] + By[jj] + cvf;
+                        const float py = fmaf(Di[ii], u1, By[jj]) + cvf;
                         w = w * sker[kb_index(py - fvj, p.fW, p.kerScale, p.kerCenter, p.K)];
-                        const float pz = Az[ii] + Bz[jj] + cvf;
+                        const float pz = fmaf(Di[ii], u2, Bz[jj]) + cvf;
                         w = w * sker[kb_index(pz - fvk, p.fW, p.kerScale, p.kerCenter, p.K)];
                         const int lin = (J - 2 + jj) * M + (I - 2 + ii);
                         if (lin >= 0 && lin < M2) {
@@ -803,11 +824,12 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BPArgs p)
                 const int W = p.convW;
                 for (int j1 = J - W; j1 <= J + W; j1++) {
                     for (int i1 = I - W; i1 <= I + W; i1++) {
-                        const float px = (float)(i1 - ci) * u0 + (float)(j1 - ci) * v0 + cvf;
+                        const float ti = (float)(i1 - ci), tj = (float)(j1 - ci);
+                        const float px = ref_sum2c(ti, u0, tj, v0, cvf);
                         float w = sker[kb_index(px - fvi, p.fW, p.kerScale, p.kerCenter, p.K)];
-                        const float py = (float)(i1 - ci) * u1 + (float)(j1 - ci) * v1 + cvf;
+                        const float py = ref_sum2c(ti, u1, tj, v1, cvf);
                         w = w * sker[kb_index(py - fvj, p.fW, p.kerScale, p.kerCenter, p.K)];
-                        const float pz = (float)(i1 - ci) * u2 + (float)(j1 - ci) * v2 + cvf;
+                        const float pz = ref_sum2c(ti, u2, tj, v2, cvf);
                         w = w * sker[kb_index(pz - fvk, p.fW, p.kerScale, p.kerCenter, p.K)];
                         const int lin = j1 * M + i1;
                         if (lin >= 0 && lin < M2) {
@@ -847,12 +869,15 @@ __global__ __launch_bounds__(kThreads) void bp_kernel(BPArgs p)
 // four wave accumulators are combined in a fixed order at the end and added
 // to the volume in FP64.
 //
-// Table index.  For every FP32 r in [-2, 2],
-// floor(fl(fl(r*125) + 250) + 0.5) equals the reference's
-// roundf(fl(fl(r*125) + 250)) (checked exhaustively), and
-// v_cvt_rpi_i32_f32 computes floor(x + 0.5) in one instruction.  The LDS table
-// is padded on both sides with copies of its end entries, so offsets beyond
-// +-2 land on the entry the reference's clamp would pick and r needs no clamp.
+// Table index.  The reference, as contracted, takes
+// k = roundf(fma(clamp(r, -2, 2), 125, 250)) with r = fl(p - v).  For every
+// FP32 r in [-4.5, 4.5], floor(fma(r, 125, 250) + 0.5) saturated to [0, 500]
+// equals k (checked exhaustively, with floor(s + 0.5) exact and with s + 0.5
+// first rounded to FP32, so the result does not depend on how the instruction
+// rounds inside), and v_cvt_rpi_i32_f32 computes floor(x + 0.5) in one
+// instruction.  The LDS table is padded on both sides with copies of its end
+// entries, so offsets beyond +-2 land on the entry the reference's clamp
+// would pick and r needs no clamp.
 //
 // Mask.  sqrtf(d2) <= maskRadius with d2 a small integer is monotone in d2,
 // so the host passes the largest passing d2 and the test is an integer
@@ -868,44 +893,31 @@ constexpr int BPCH = 128;              // directions culled per pass (queue entr
 
 __device__ __forceinline__ float kb_fast(const float *__restrict__ sk, float r)
 {
-    float s = r * 125.0f;
-    s = s + 250.0f;
-    int i = cvt_rpi(s);  // floor(s + 0.5)
+    int i = cvt_rpi(fmaf(r, 125.0f, 250.0f));  // floor(s + 0.5)
     i = min(max(i, -KPAD), KFAST - 1 + KPAD);
     return sk[KPAD + i];
 }
 
 // Same lookup without the clamp, for directions whose frame is orthonormal
 // (checked per chunk): then every tap has |r| <= 4.13 and the index stays in
-// [-267, 767], inside the padding (see bp_fast_kernel).
+// [-267, 767], inside the padding.
 __device__ __forceinline__ float kb_fast_nc(const float *__restrict__ sk, float r)
 {
-    float s = r * 125.0f;
-    s = s + 250.0f;
-    return sk[KPAD + cvt_rpi(s)];
+    return sk[KPAD + cvt_rpi(fmaf(r, 125.0f, 250.0f))];
 }
 
-// Fused forms of the same unclamped lookup for a tap at position p (the
-// reference's fl(fl(A + B) + volCenter)) and an integer voxel coordinate v,
-// |p - v| <= 4.5.  The reference takes r = fl(p - v), fl(r * 125), + 250.
-//  * v >= 9: p - v is exact (Sterbenz), so fl(r * 125) = fma(p, 125, -125 v)
-//    (125 v is an exact integer); form B = fl(fma(p, 125, -125 v) + 250).
-//  * v >= 130: every tap with |r| <= 2 has p >= 128, where r is a multiple of
-//    2^-16 or coarser, so 125 r is exact and the two roundings collapse into
-//    one; taps with |r| > 2 saturate at table entry 0 or 500 in either form.
-//    Form A = fma(p, 125, 250 - 125 v).
-// Both were checked against the reference's clamped index for every float p
-// in [v - 4.5, v + 4.5] and every integer v up to 2,060: 0 mismatches.  The
-// caller passes c = -125 v (form B) or 250 - 125 v (form A).
-constexpr int BP_FORM_B_MIN = 9;
-constexpr int BP_FORM_A_MIN = 130;
+// Fused form of the same unclamped lookup for a tap at position p and an
+// integer voxel coordinate v >= 3: where the reference's clamp is inactive
+// (|p - v| <= 2) p >= 1, so p and v are multiples of ulp(p) and p - v is
+// exact; then fma(r, 125, 250) = RN(125 p + 250 - 125 v) = fma(p, 125, c) with
+// c = 250 - 125 v an exact integer.  Where |p - v| > 2 both sides saturate at
+// table entry 0 or 500.  Checked for every float p in [v - 4.5, v + 4.5] and
+// v in 3..4 and 9..2,100 (0 mismatches); v = 0, 1, 2 have mismatches.
+constexpr int BP_FORM_A_MIN = 3;
 
-template <int FORM>
 __device__ __forceinline__ float kb_form(const float *__restrict__ sk, float pos, float c)
 {
-    float s = fmaf(pos, 125.0f, c);
-    if (FORM == 1) s = s + 250.0f;
-    return sk[KPAD + cvt_rpi(s)];
+    return sk[KPAD + cvt_rpi(fmaf(pos, 125.0f, c))];
 }
 
 __device__ __forceinline__ void wave_lds_sync()
@@ -949,11 +961,11 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
         if (DENS) (&sden[0][0])[k] = 0.f;
     }
 
-    // Lookup form per axis for orthonormal chunks (kb_form): 2 = A, 1 = B, 0 =
-    // the unfused lookup; the brick's smallest coordinate on the axis decides.
-    auto formOf = [](int o) { return o >= BP_FORM_A_MIN ? 2 : (o >= BP_FORM_B_MIN ? 1 : 0); };
-    const int fx = formOf(ox), fy = formOf(oy), fz = formOf(oz);
-    const int brickForm = (p.posRound && fx && fy && fz) ? fx + 2 * (fy - 1) + 4 * (fz - 1) : 0;
+    // Orthonormal chunks use the fused lookup (kb_form) when every active
+    // voxel of the brick has coordinates >= BP_FORM_A_MIN: the brick's corner
+    // does, or the mask keeps every in-mask voxel there.
+    const bool brickA = p.posRound &&
+                        (min(ox, min(oy, oz)) >= BP_FORM_A_MIN || p.maskLo >= BP_FORM_A_MIN);
 
     const float cvf = (float)cv, cif = (float)ci;
     const float bc = 0.5f * (BB - 1);
@@ -1024,13 +1036,14 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
             if (q < total) {
                 const float *ax = sax + q * 9;
                 const float n0 = ax[6], n1 = ax[7], n2 = ax[8];
-                // Slab test of the lane's column (pinned FP32 order), appended
-                // to the queue in (z, lane) order.
-                const float fk0 = dvi0 * n0 + dvj0 * n1;
+                // Slab test of the lane's column (the reference's contracted
+                // fk = fma(dvk, n2, fma(dvi, n0, fl(dvj * n1)))), appended to
+                // the queue in (z, lane) order.
+                const float fk0 = fmaf(dvi0, n0, dvj0 * n1);
 #pragma unroll
                 for (int k = 0; k < BB; k++) {
                     const float dvk = (float)(oz + k) - cv;
-                    const float fk = fk0 + dvk * n2;
+                    const float fk = fmaf(dvk, n2, fk0);
                     const bool on = ((vmask >> k) & 1u) && fabsf(fk) <= p.hw;
                     const unsigned long long b = __ballot(on);
                     if (on)
@@ -1056,27 +1069,21 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                     const int lx = vox & 7, ly = (vox >> 3) & 7, lz = vox >> 6;
                     const int vi = ox + lx, vj = oy + ly, vk = oz + lz;
                     const float dvi = (float)vi - cv, dvj = (float)vj - cv, dvk = (float)vk - cv;
-                    const float fi = dvi * u0 + dvj * u1 + dvk * u2 + cif;
-                    const float fj = dvi * v0 + dvj * v1 + dvk * v2 + cif;
+                    const float fi = ref_sum3(dvi, u0, dvj, u1, dvk, u2) + cif;
+                    const float fj = ref_sum3(dvi, v0, dvj, v1, dvk, v2) + cif;
                     constexpr int KF = decltype(formT)::value;
-                    // The fused variants run only when posRound holds (fi, fj >= 0.5),
+                    // The fused variant runs only when posRound holds (fi, fj >= 0.5),
                     // where floor(x + 0.5) is roundf(x) and one instruction.
                     const int I = KF > 0 ? cvt_rpi(fi) : (int)roundf(fi);
                     const int J = KF > 0 ? cvt_rpi(fj) : (int)roundf(fj);
                     const float fvi = (float)vi, fvj = (float)vj, fvk = (float)vk;
-                    // Form constants (used only by the fused variants; exact).
-                    constexpr int FX = KF > 0 ? 1 + ((KF - 1) & 1) : 0;
-                    constexpr int FY = KF > 0 ? 1 + (((KF - 1) >> 1) & 1) : 0;
-                    constexpr int FZ = KF > 0 ? 1 + (((KF - 1) >> 2) & 1) : 0;
-                    const float cX = FX == 2 ? fmaf(-125.0f, fvi, 250.0f) : -125.0f * fvi;
-                    const float cY = FY == 2 ? fmaf(-125.0f, fvj, 250.0f) : -125.0f * fvj;
-                    const float cZ = FZ == 2 ? fmaf(-125.0f, fvk, 250.0f) : -125.0f * fvk;
-                    float Ax[5], Ay[5], Az[5];
+                    // Form constants 250 - 125 v (exact; used by the fused variant).
+                    const float cX = fmaf(-125.0f, fvi, 250.0f);
+                    const float cY = fmaf(-125.0f, fvj, 250.0f);
+                    const float cZ = fmaf(-125.0f, fvk, 250.0f);
+                    float Di[5];
 #pragma unroll
-                    for (int s5 = 0; s5 < 5; s5++) {
-                        const float di = (float)(I - 2 + s5 - ci);
-                        Ax[s5] = di * u0; Ay[s5] = di * u1; Az[s5] = di * u2;
-                    }
+                    for (int s5 = 0; s5 < 5; s5++) Di[s5] = (float)(I - 2 + s5 - ci);
                     float cum = 0.f, dsum = 0.f;
                     const bool inside = I >= 2 && I + 2 < M && J >= 2 && J + 2 < M;
 #pragma unroll 1
@@ -1101,9 +1108,11 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                         }
 #pragma unroll
                         for (int ii = 0; ii < 5; ii++) {
-                            const float px = Ax[ii] + Bx + cvf;
-                            const float py = Ay[ii] + By + cvf;
-                            const float pz = Az[ii] + Bz + cvf;
+                            // fl(fma(di, u, fl(dj*v)) + c), the reference's contracted
+                            // tap position (gpuBackProject.cu:77,84,91).
+                            const float px = fmaf(Di[ii], u0, Bx) + cvf;
+                            const float py = fmaf(Di[ii], u1, By) + cvf;
+                            const float pz = fmaf(Di[ii], u2, Bz) + cvf;
                             float w, wy, wz;
                             if constexpr (KF < 0) {
                                 w = kb_fast(sker, px - fvi);
@@ -1114,9 +1123,9 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                                 wy = kb_fast_nc(sker, py - fvj);
                                 wz = kb_fast_nc(sker, pz - fvk);
                             } else {
-                                w = kb_form<FX>(sker, px, cX);
-                                wy = kb_form<FY>(sker, py, cY);
-                                wz = kb_form<FZ>(sker, pz, cZ);
+                                w = kb_form(sker, px, cX);
+                                wy = kb_form(sker, py, cY);
+                                wz = kb_form(sker, pz, cZ);
                             }
                             w = w * wy;
                             w = w * wz;
@@ -1136,18 +1145,10 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
             };
             if (!chunkOrtho) {
                 gather(std::integral_constant<int, -1>{});
+            } else if (brickA) {
+                gather(std::integral_constant<int, 1>{});
             } else {
-                switch (brickForm) {
-                case 1: gather(std::integral_constant<int, 1>{}); break;
-                case 2: gather(std::integral_constant<int, 2>{}); break;
-                case 3: gather(std::integral_constant<int, 3>{}); break;
-                case 4: gather(std::integral_constant<int, 4>{}); break;
-                case 5: gather(std::integral_constant<int, 5>{}); break;
-                case 6: gather(std::integral_constant<int, 6>{}); break;
-                case 7: gather(std::integral_constant<int, 7>{}); break;
-                case 8: gather(std::integral_constant<int, 8>{}); break;
-                default: gather(std::integral_constant<int, 0>{}); break;
-                }
+                gather(std::integral_constant<int, 0>{});
             }
             wave_lds_sync();  // queue slots consumed before they are refilled
             if (last) break;
@@ -2216,6 +2217,16 @@ inline int mask_d2(float r, int v)
     return (int)lo;
 }
 
+// The mask radius as the reference's kernels see it: RunKernel takes it as an
+// int (gpuForwardProject.h:57, gpuBackProject.h:56) from the float member
+// (gpuGridder.cpp:262,422), so it is truncated toward zero.  NaN and values
+// beyond the int range (no meaningful conversion) pass through.
+inline float ref_mask_radius(float r)
+{
+    if (!(r > -2147483648.0f && r < 2147483648.0f)) return r;
+    return (float)(int)r;
+}
+
 inline void proj_consts(const ProjGeom &g, float &kerScale, float &kerCenter, float &fW,
                         int &convW)
 {
@@ -2238,7 +2249,7 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
     FPArgs a;
     a.vol = vol; a.axes = axes; a.out = casImgs; a.ker = g.ker;
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
-    a.maskR = g.maskRadius;
+    a.maskR = ref_mask_radius(g.maskRadius);
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
     const bool w2 = a.convW == 2 && g.K <= FPKER;
     if (w2) {
@@ -2246,7 +2257,7 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
         const bool fast = a.kerScale == 125.0f && a.kerCenter == 250.0f && g.K == 501;
         a.tilesPerRow = (g.M + FPT - 1) / FPT;
         a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
-        a.maskD2 = mask_d2(g.maskRadius, g.V);
+        a.maskD2 = mask_d2(a.maskR, g.V);
         const unsigned long long work = (unsigned long long)a.tilesPerImg * nAxes;
         if (work > 0xffffffffull / 2) return hipErrorInvalidValue;
         a.totalWork = (unsigned)work;
@@ -2335,7 +2346,7 @@ hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const f
     FPArgs a;
     a.vol = nullptr; a.axes = axes; a.out = nullptr; a.ker = g.ker;
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
-    a.maskR = g.maskRadius;
+    a.maskR = ref_mask_radius(g.maskRadius);
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
     a.tilesPerRow = (g.M + 15) / 16;
     a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
@@ -2355,15 +2366,20 @@ hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
     BPArgs a;
     a.vol = volAcc; a.dens = densAcc; a.axes = axes; a.img = casImgs; a.ker = g.ker;
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
-    a.maskR = g.maskRadius; a.hw = g.kerHWidth;
+    a.maskR = ref_mask_radius(g.maskRadius); a.hw = g.kerHWidth;
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
     const bool fast = bp_uses_fast_path(g);
-    a.maskD2 = mask_d2(g.maskRadius, g.V);
+    a.maskD2 = mask_d2(a.maskR, g.V);
+    {
+        int r = 0;  // floor(sqrt(maskD2))
+        while ((long long)(r + 1) * (r + 1) <= (long long)a.maskD2) r++;
+        a.maskLo = a.maskD2 < 0 ? g.V : g.V / 2 - r;
+    }
     // In-mask voxel: |v - c| <= maskR; orthonormal frame (to 1e-4): |u|, |v| <=
     // 1.00005, and the FP32 evaluation of fi adds < 1e-3.  Then fi, fj >=
     // M/2 - 1.00005 maskR - 1e-3, which must be >= 0.5 for floor(fi + 0.5) to be
     // roundf(fi) (for 0.25 <= x < 0.5 an FP32 x + 0.5 can round up to 1).
-    a.posRound = ((double)(g.M / 2) - 1.00005 * (double)g.maskRadius - 1e-3 >= 0.5) ? 1 : 0;
+    a.posRound = ((double)(g.M / 2) - 1.00005 * (double)a.maskR - 1e-3 >= 0.5) ? 1 : 0;
     const int ex = fast ? BB : BPX, ey = fast ? BB : BPY, ez = fast ? BB : BPZ;
     a.bricksX = (g.V + ex - 1) / ex;
     const int bricksY = (g.V + ey - 1) / ey;

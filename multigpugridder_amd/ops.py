@@ -42,6 +42,41 @@ def _check(t: torch.Tensor, name: str, dtype=torch.float32):
         raise ValueError(f"{name} must be contiguous")
 
 
+# Shape checks run before the kernels see a pointer: a kernel trusts the sizes
+# it is given, so a tensor smaller than what it reads or writes would be an
+# out-of-bounds device access.  They come before the device checks so that
+# they can be exercised without a GPU.
+def _need_numel(t: torch.Tensor, n: int, name: str, exact: bool = True):
+    if (t.numel() != n) if exact else (t.numel() < n):
+        what = "exactly" if exact else "at least"
+        raise ValueError(f"{name} must hold {what} {n} elements, got {t.numel()} (shape {tuple(t.shape)})")
+
+
+def _cube_edge(t: torch.Tensor, name: str) -> int:
+    if t.dim() != 3 or not (t.shape[0] == t.shape[1] == t.shape[2]) or t.shape[0] == 0:
+        raise ValueError(f"{name} must be a non-empty cube (V, V, V), got shape {tuple(t.shape)}")
+    return int(t.shape[0])
+
+
+def _stack_edge(t: torch.Tensor, name: str) -> tuple:
+    """(P, S, S) stack of square images -> (P, S)."""
+    if t.dim() != 3 or t.shape[1] != t.shape[2] or t.shape[1] == 0:
+        raise ValueError(f"{name} must be a stack of square images (P, S, S), got shape {tuple(t.shape)}")
+    return int(t.shape[0]), int(t.shape[1])
+
+
+def _n_axes(axes: torch.Tensor) -> int:
+    if axes.numel() % 9:
+        raise ValueError(f"axes must hold 9 floats per direction, got {axes.numel()} elements")
+    return axes.numel() // 9
+
+
+def _same_device(*named):
+    devs = {(n, t.device) for n, t in named if t is not None}
+    if len({d for _, d in devs}) > 1:
+        raise ValueError("tensors on different devices: " + ", ".join(f"{n} on {d}" for n, d in sorted(devs, key=str)))
+
+
 def kb_table(device=None, hwidth: int = 2, size: int = 501) -> torch.Tensor:
     """Device copy of the Kaiser-Bessel table (cached per device)."""
     dev = torch.device("cuda", torch.cuda.current_device()) if device is None else torch.device(device)
@@ -65,16 +100,20 @@ def forward_project(cas_volume: torch.Tensor, axes: torch.Tensor, m: int, mask_r
     cas_volume may be float32, bfloat16 or float16 (gather in FP32)."""
     if cas_volume.dtype not in _DTYPES:
         raise ValueError(f"unsupported CAS volume dtype {cas_volume.dtype}")
+    v = _cube_edge(cas_volume, "cas_volume")
+    p = _n_axes(axes)
+    if int(m) <= 0:
+        raise ValueError(f"image size m must be positive, got {m}")
+    if out is not None:
+        _need_numel(out, p * m * m, "out")
+    _same_device(("cas_volume", cas_volume), ("axes", axes), ("out", out), ("ker", ker))
     _check(cas_volume, "cas_volume", None)
     _check(axes, "axes")
-    v = cas_volume.shape[-1]
-    p = axes.numel() // 9
     if out is None:
         out = torch.empty((p, m, m), device=cas_volume.device, dtype=torch.float32)
     _check(out, "out")
-    if out.numel() < p * m * m:
-        raise ValueError("out too small")
     ker = kb_table(cas_volume.device) if ker is None else ker
+    _check(ker, "ker")
     mr = default_mask_radius(m) if mask_radius is None else float(mask_radius)
     _lib.call("mgg_op_forward_project", _ptr(cas_volume), _DTYPES[cas_volume.dtype], v, _ptr(axes), p,
               _ptr(out), m, mr, _ptr(ker), ker.numel(), float(hwidth), _stream())
@@ -85,18 +124,26 @@ def back_project(cas_images: torch.Tensor, axes: torch.Tensor, v: int, mask_radi
                  ker: torch.Tensor | None = None, hwidth: float = 2.0, out: torch.Tensor | None = None,
                  density: torch.Tensor | None = None) -> torch.Tensor:
     """Reference-semantics back projection, accumulated into ``out`` (V^3, zeroed if new)."""
+    pi, m = _stack_edge(cas_images, "cas_images")
+    p = _n_axes(axes)
+    if pi < p:
+        raise ValueError(f"cas_images holds {pi} images for {p} directions")
+    if int(v) <= 0:
+        raise ValueError(f"volume size v must be positive, got {v}")
+    if out is not None:
+        _need_numel(out, v * v * v, "out")
+    if density is not None:
+        _need_numel(density, v * v * v, "density")
+    _same_device(("cas_images", cas_images), ("axes", axes), ("out", out), ("density", density), ("ker", ker))
     _check(cas_images, "cas_images")
     _check(axes, "axes")
-    m = cas_images.shape[-1]
-    p = axes.numel() // 9
-    if cas_images.numel() < p * m * m:
-        raise ValueError("cas_images holds fewer images than axes")
     if out is None:
         out = torch.zeros((v, v, v), device=cas_images.device, dtype=torch.float32)
     _check(out, "out")
     if density is not None:
         _check(density, "density")
     ker = kb_table(cas_images.device) if ker is None else ker
+    _check(ker, "ker")
     mr = default_mask_radius(m) if mask_radius is None else float(mask_radius)
     _lib.call("mgg_op_back_project", _ptr(out), _ptr(density) if density is not None else None, v,
               _ptr(axes), p, _ptr(cas_images), m, mr, _ptr(ker), ker.numel(), float(hwidth), _stream())
@@ -107,14 +154,22 @@ def back_project_adjoint(cas_images: torch.Tensor, axes: torch.Tensor, v: int, m
                          ker: torch.Tensor | None = None, hwidth: float = 2.0,
                          out: torch.Tensor | None = None) -> torch.Tensor:
     """Exact transpose of :func:`forward_project`, accumulated into ``out``."""
+    pi, m = _stack_edge(cas_images, "cas_images")
+    p = _n_axes(axes)
+    if pi < p:
+        raise ValueError(f"cas_images holds {pi} images for {p} directions")
+    if int(v) <= 0:
+        raise ValueError(f"volume size v must be positive, got {v}")
+    if out is not None:
+        _need_numel(out, v * v * v, "out")
+    _same_device(("cas_images", cas_images), ("axes", axes), ("out", out), ("ker", ker))
     _check(cas_images, "cas_images")
     _check(axes, "axes")
-    m = cas_images.shape[-1]
-    p = axes.numel() // 9
     if out is None:
         out = torch.zeros((v, v, v), device=cas_images.device, dtype=torch.float32)
     _check(out, "out")
     ker = kb_table(cas_images.device) if ker is None else ker
+    _check(ker, "ker")
     mr = default_mask_radius(m) if mask_radius is None else float(mask_radius)
     _lib.call("mgg_op_back_project_adjoint", _ptr(out), v, _ptr(axes), p, _ptr(cas_images), m, mr,
               _ptr(ker), ker.numel(), float(hwidth), _stream())
@@ -125,8 +180,13 @@ def cas_images_to_images(cas_images: torch.Tensor, n: int, out: torch.Tensor | N
                          clobber: bool = False) -> torch.Tensor:
     """(P, m, m) CAS images -> (P, n, n) images.  The kernel uses its input as
     scratch; unless ``clobber`` it works on a copy."""
+    p, m = _stack_edge(cas_images, "cas_images")
+    if not 0 < n <= m:
+        raise ValueError(f"image size n must be in [1, {m}], got {n}")
+    if out is not None:
+        _need_numel(out, p * n * n, "out")
+    _same_device(("cas_images", cas_images), ("out", out))
     _check(cas_images, "cas_images")
-    p, m = cas_images.shape[0], cas_images.shape[-1]
     src = cas_images if clobber else cas_images.clone()
     if out is None:
         out = torch.empty((p, n, n), device=cas_images.device, dtype=torch.float32)
@@ -136,8 +196,13 @@ def cas_images_to_images(cas_images: torch.Tensor, n: int, out: torch.Tensor | N
 
 
 def images_to_cas_images(images: torch.Tensor, m: int, out: torch.Tensor | None = None) -> torch.Tensor:
+    p, n = _stack_edge(images, "images")
+    if m < n:
+        raise ValueError(f"CAS image size m must be >= {n}, got {m}")
+    if out is not None:
+        _need_numel(out, p * m * m, "out")
+    _same_device(("images", images), ("out", out))
     _check(images, "images")
-    p, n = images.shape[0], images.shape[-1]
     if out is None:
         out = torch.empty((p, m, m), device=images.device, dtype=torch.float32)
     _check(out, "out")
@@ -147,10 +212,13 @@ def images_to_cas_images(images: torch.Tensor, m: int, out: torch.Tensor | None 
 
 def volume_to_cas_volume(volume: torch.Tensor, interp_factor: float = 2.0, extra_padding: int = 3,
                          hwidth: float = 2.0, out: torch.Tensor | None = None) -> torch.Tensor:
-    _check(volume, "volume")
-    n = volume.shape[-1]
+    n = _cube_edge(volume, "volume")
     m = int(np.float32(n) * np.float32(interp_factor))
     v = m + 2 * extra_padding
+    if out is not None:
+        _need_numel(out, v * v * v, "out")
+    _same_device(("volume", volume), ("out", out))
+    _check(volume, "volume")
     if out is None:
         out = torch.empty((v, v, v), device=volume.device, dtype=torch.float32)
     _check(out, "out")
@@ -162,8 +230,12 @@ def volume_to_cas_volume(volume: torch.Tensor, interp_factor: float = 2.0, extra
 def cas_volume_to_volume(cas_volume: torch.Tensor, n: int, interp_factor: float = 2.0,
                          extra_padding: int = 3, hwidth: float = 2.0, zero_nyquist: bool = True,
                          out: torch.Tensor | None = None) -> torch.Tensor:
-    _check(cas_volume, "cas_volume")
     m = int(np.float32(n) * np.float32(interp_factor))
+    _need_numel(cas_volume, (m + 2 * extra_padding) ** 3, "cas_volume")
+    if out is not None:
+        _need_numel(out, n * n * n, "out")
+    _same_device(("cas_volume", cas_volume), ("out", out))
+    _check(cas_volume, "cas_volume")
     if out is None:
         out = torch.empty((n, n, n), device=cas_volume.device, dtype=torch.float32)
     _check(out, "out")
@@ -176,8 +248,13 @@ def cas_images_to_images_transpose(images: torch.Tensor, m: int,
                                    out: torch.Tensor | None = None) -> torch.Tensor:
     """Transpose of :func:`cas_images_to_images`: (P, n, n) -> (P, m, m),
     (1/m^2) ImgsToCAS with the index-0 row / column of each CAS image zeroed."""
+    p, n = _stack_edge(images, "images")
+    if m < n:
+        raise ValueError(f"CAS image size m must be >= {n}, got {m}")
+    if out is not None:
+        _need_numel(out, p * m * m, "out")
+    _same_device(("images", images), ("out", out))
     _check(images, "images")
-    p, n = images.shape[0], images.shape[-1]
     if out is None:
         out = torch.empty((p, m, m), device=images.device, dtype=torch.float32)
     _check(out, "out")
@@ -190,8 +267,12 @@ def volume_to_cas_volume_transpose(cas_volume: torch.Tensor, n: int, interp_fact
                                    out: torch.Tensor | None = None) -> torch.Tensor:
     """Transpose of :func:`volume_to_cas_volume`: (V^3) -> (n^3), m^3 CASToVol
     keeping the index-0 planes."""
-    _check(cas_volume, "cas_volume")
     m = int(np.float32(n) * np.float32(interp_factor))
+    _need_numel(cas_volume, (m + 2 * extra_padding) ** 3, "cas_volume")
+    if out is not None:
+        _need_numel(out, n * n * n, "out")
+    _same_device(("cas_volume", cas_volume), ("out", out))
+    _check(cas_volume, "cas_volume")
     if out is None:
         out = torch.empty((n, n, n), device=cas_volume.device, dtype=torch.float32)
     _check(out, "out")
@@ -202,18 +283,28 @@ def volume_to_cas_volume_transpose(cas_volume: torch.Tensor, n: int, interp_fact
 
 def divide_density(num: torch.Tensor, den: torch.Tensor, eps: float = 1e-6,
                    out: torch.Tensor | None = None) -> torch.Tensor:
+    _need_numel(den, num.numel(), "den")
+    if out is not None:
+        _need_numel(out, num.numel(), "out")
+    _same_device(("num", num), ("den", den), ("out", out))
     _check(num, "num")
     _check(den, "den")
     out = torch.empty_like(num) if out is None else out
+    _check(out, "out")
     _lib.call("mgg_op_divide_density", _ptr(out), _ptr(num), _ptr(den), num.numel(), float(eps), _stream())
     return out
 
 
 def reduce_partials(parts: torch.Tensor, out: torch.Tensor | None = None, accumulate: bool = False):
     """Sum (K, ...) partial volumes along dim 0 in FP64 -> FP32."""
-    _check(parts, "parts")
+    if parts.dim() < 2 or parts.shape[0] == 0:
+        raise ValueError(f"parts must be (K, ...) with K >= 1, got shape {tuple(parts.shape)}")
     k = parts.shape[0]
     n = parts[0].numel()
+    if out is not None:
+        _need_numel(out, n, "out")
+    _same_device(("parts", parts), ("out", out))
+    _check(parts, "parts")
     if out is None:
         out = torch.empty(parts.shape[1:], device=parts.device, dtype=torch.float32)
     _check(out, "out")

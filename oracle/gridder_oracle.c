@@ -17,11 +17,30 @@
  *
  * Numerics contract.  Every quantity that feeds a discrete decision (the
  * rotated coordinate, roundf of it, the kernel-table index, the mask test and
- * the back-projection slab gate) is evaluated in FP32 with the operation order
- * written in the reference source and with NO fused multiply-add contraction
- * (this file must be compiled with -ffp-contract=off and without -ffast-math;
- * oracle/Makefile does so).  The HIP kernels follow the same pinned order, so
- * both sides take identical decisions.  Accumulation here is FP64.
+ * the back-projection slab gate) is evaluated in FP32 under one "reading" of
+ * the reference source: the reference is built by nvcc with no flags
+ * (src/CMakeLists.txt:14-16,80-83), and nvcc's default --fmad=true contracts a
+ * multiply feeding an add into one fused multiply-add, so the source text
+ * alone does not fix the arithmetic.  The readings restated here:
+ *   ORC_READING_SOURCE  : no contraction, the source's operation order;
+ *   ORC_READING_FIRST   : (default) contraction without reassociation, each
+ *                         add of two products fusing its first operand:
+ *                         a*x + b*y + c  -> fl(fma(a, x, fl(b*y)) + c),
+ *                         A + B + C (+ c)-> fma(C, fma(A, fl(B))) (+ c),
+ *                         r*kerScale + kerCenter -> fma(r, kerScale, kerCenter);
+ *   ORC_READING_SECOND  : as FIRST with the second product fused instead;
+ *   ORC_READING_REASSOC : FIRST plus the nested-FMA reassociation older LLVM
+ *                         DAG combiners applied under contraction alone:
+ *                         a*x + b*y + c -> fma(a, x, fma(b, y, c)),
+ *                         A + B + C + c -> fma(C, fma(A, fma(B, c)));
+ *   ORC_READING_INDEX   : only the table index fused (sums as SOURCE).
+ * This file must be compiled with -ffp-contract=off and without -ffast-math
+ * (oracle/Makefile does so): every fused operation is written as fmaf.
+ * Accumulation here is FP64.
+ *
+ * The mask radius reaches the reference kernels through an int parameter
+ * (gpuForwardProject.h:57, gpuBackProject.h:56; gpuGridder.cpp:262,422 pass
+ * the float member), so every entry point truncates it toward zero first.
  *
  * Array layout (all arrays): column-major with x fastest,
  *   vol[x + y*V + z*V*V], img[a*M*M + i + j*M]   (gpuForwardProject.cu:90-93).
@@ -36,14 +55,88 @@
 #include <omp.h>
 #endif
 
+enum {
+    ORC_READING_SOURCE = 0,
+    ORC_READING_FIRST = 1,
+    ORC_READING_SECOND = 2,
+    ORC_READING_REASSOC = 3,
+    ORC_READING_INDEX = 4,
+};
+static int g_reading = ORC_READING_FIRST;
+
+/* Selects the arithmetic reading (see the header); returns the previous one,
+ * or -1 for an unknown value (then nothing changes). */
+int orc_set_reading(int reading)
+{
+    if (reading < ORC_READING_SOURCE || reading > ORC_READING_INDEX) return -1;
+    int prev = g_reading;
+    g_reading = reading;
+    return prev;
+}
+
+int orc_get_reading(void) { return g_reading; }
+
+/* The int conversion of the mask radius (gpuForwardProject.h:57). */
+static inline float ref_mask_radius(float r)
+{
+    if (!(r > -2147483648.0f && r < 2147483648.0f)) return r;  /* NaN, huge: no int */
+    return (float)(int)r;
+}
+
+/* a*x + b*y + c: a pixel's rotated coordinate (gpuForwardProject.cu:53-55) and
+ * a back-projection tap's position (gpuBackProject.cu:77,84,91). */
+static inline float sum2c(float a, float x, float b, float y, float c)
+{
+    switch (g_reading) {
+    case ORC_READING_FIRST: return fmaf(a, x, b * y) + c;
+    case ORC_READING_SECOND: return fmaf(b, y, a * x) + c;
+    case ORC_READING_REASSOC: return fmaf(a, x, fmaf(b, y, c));
+    default: {
+        float p = a * x, q = b * y;
+        float s = p + q;
+        return s + c;
+    }
+    }
+}
+
+/* x0*a0 + x1*a1 + x2*a2, then + c when addC: a voxel's image-frame coordinate
+ * (gpuBackProject.cu:62-64; fk has no constant). */
+static inline float sum3c(float x0, float a0, float x1, float a1, float x2, float a2, float c, int addC)
+{
+    switch (g_reading) {
+    case ORC_READING_FIRST: {
+        float s = fmaf(x2, a2, fmaf(x0, a0, x1 * a1));
+        return addC ? s + c : s;
+    }
+    case ORC_READING_SECOND: {
+        float s = fmaf(x2, a2, fmaf(x1, a1, x0 * a0));
+        return addC ? s + c : s;
+    }
+    case ORC_READING_REASSOC:
+        if (addC) return fmaf(x2, a2, fmaf(x0, a0, fmaf(x1, a1, c)));
+        return fmaf(x2, a2, fmaf(x0, a0, x1 * a1));
+    default: {
+        float p = x0 * a0, q = x1 * a1, t = x2 * a2;
+        float s = p + q;
+        s = s + t;
+        return addC ? s + c : s;
+    }
+    }
+}
+
 /* Kernel-table lookup index for a signed offset r (gpuForwardProject.cu:63-67,
- * gpuBackProject.cu:79-82): clamp r to [-convW, convW], scale, round half
- * away from zero, clamp to the table. */
+ * gpuBackProject.cu:79-82): clamp r to [-convW, convW], scale and offset,
+ * round half away from zero, clamp to the table. */
 static inline int kb_index(float r, float convW, float kerScale, float kerCenter, int K)
 {
     r = fminf(fmaxf(r, -convW), convW);
-    float s = r * kerScale;
-    s = s + kerCenter;
+    float s;
+    if (g_reading == ORC_READING_SOURCE) {
+        s = r * kerScale;
+        s = s + kerCenter;
+    } else {
+        s = fmaf(r, kerScale, kerCenter);
+    }
     int idx = (int)roundf(s);
     if (idx < 0) idx = 0;
     if (idx > K - 1) idx = K - 1;
@@ -57,6 +150,7 @@ void orc_forward_project(const float *vol, int V, const float *axes, int nAxes,
                          const float *ker, int K, float kerHWidth, int M,
                          float maskRadius, double *out)
 {
+    maskRadius = ref_mask_radius(maskRadius);
     const int cv = V / 2, ci = M / 2;
     const int convW = (int)roundf(kerHWidth);
     const float fW = (float)convW;
@@ -76,9 +170,9 @@ void orc_forward_project(const float *vol, int V, const float *axes, int nAxes,
                 double acc = 0.0;
                 if (r <= maskRadius) {
                     float fi = (float)(i - ci), fj = (float)(j - ci);
-                    float x = nx[0] * fi + ny[0] * fj + (float)cv;
-                    float y = nx[1] * fi + ny[1] * fj + (float)cv;
-                    float z = nx[2] * fi + ny[2] * fj + (float)cv;
+                    float x = sum2c(nx[0], fi, ny[0], fj, (float)cv);
+                    float y = sum2c(nx[1], fi, ny[1], fj, (float)cv);
+                    float z = sum2c(nx[2], fi, ny[2], fj, (float)cv);
                     int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
                     for (int i1 = X - convW; i1 <= X + convW; i1++) {
                         float wi = ker[kb_index((float)i1 - x, fW, kerScale, kerCenter, K)];
@@ -107,6 +201,7 @@ void orc_back_project_adjoint(double *volOut, int V, const float *axes, int nAxe
                               const float *img, int M, float maskRadius,
                               const float *ker, int K, float kerHWidth)
 {
+    maskRadius = ref_mask_radius(maskRadius);
     const int cv = V / 2, ci = M / 2;
     const int convW = (int)roundf(kerHWidth);
     const float fW = (float)convW;
@@ -125,9 +220,9 @@ void orc_back_project_adjoint(double *volOut, int V, const float *axes, int nAxe
                 double p = (double)im[(long long)j * M + i];
                 if (p == 0.0) continue;
                 float fi = (float)(i - ci), fj = (float)(j - ci);
-                float x = nx[0] * fi + ny[0] * fj + (float)cv;
-                float y = nx[1] * fi + ny[1] * fj + (float)cv;
-                float z = nx[2] * fi + ny[2] * fj + (float)cv;
+                float x = sum2c(nx[0], fi, ny[0], fj, (float)cv);
+                float y = sum2c(nx[1], fi, ny[1], fj, (float)cv);
+                float z = sum2c(nx[2], fi, ny[2], fj, (float)cv);
                 int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
                 for (int i1 = X - convW; i1 <= X + convW; i1++) {
                     float wi = ker[kb_index((float)i1 - x, fW, kerScale, kerCenter, K)];
@@ -157,6 +252,7 @@ void orc_back_project(double *volOut, int V, const float *axes, int nAxes,
                       const float *img, int M, float maskRadius,
                       const float *ker, int K, float kerHWidth)
 {
+    maskRadius = ref_mask_radius(maskRadius);
     const int cv = V / 2, ci = M / 2;
     const int convW = (int)roundf(kerHWidth);
     const float fW = (float)convW;
@@ -176,20 +272,22 @@ void orc_back_project(double *volOut, int V, const float *axes, int nAxes,
             const float *nx = axes + 9 * a;
             const float *ny = nx + 3;
             const float *nz = ny + 3;
-            float fi = ((float)vi - cv) * nx[0] + ((float)vj - cv) * nx[1] + ((float)vk - cv) * nx[2] + ci;
-            float fj = ((float)vi - cv) * ny[0] + ((float)vj - cv) * ny[1] + ((float)vk - cv) * ny[2] + ci;
-            float fk = ((float)vi - cv) * nz[0] + ((float)vj - cv) * nz[1] + ((float)vk - cv) * nz[2];
+            const float di = (float)vi - cv, dj = (float)vj - cv, dk = (float)vk - cv;
+            float fi = sum3c(di, nx[0], dj, nx[1], dk, nx[2], (float)ci, 1);
+            float fj = sum3c(di, ny[0], dj, ny[1], dk, ny[2], (float)ci, 1);
+            float fk = sum3c(di, nz[0], dj, nz[1], dk, nz[2], 0.f, 0);
             if (!(fabsf(fk) <= kerHWidth)) continue;
             int I = (int)roundf(fi), J = (int)roundf(fj);
             const float *im = img + (long long)a * M2;
             double cum = 0.0;
             for (int j1 = J - convW; j1 <= J + convW; j1++) {
                 for (int i1 = I - convW; i1 <= I + convW; i1++) {
-                    float px = (i1 - ci) * nx[0] + (j1 - ci) * ny[0] + cv;
+                    const float ti = (float)(i1 - ci), tj = (float)(j1 - ci);
+                    float px = sum2c(ti, nx[0], tj, ny[0], (float)cv);
                     float w = ker[kb_index(px - vi, fW, kerScale, kerCenter, K)];
-                    float py = (i1 - ci) * nx[1] + (j1 - ci) * ny[1] + cv;
+                    float py = sum2c(ti, nx[1], tj, ny[1], (float)cv);
                     w = w * ker[kb_index(py - vj, fW, kerScale, kerCenter, K)];
-                    float pz = (i1 - ci) * nx[2] + (j1 - ci) * ny[2] + cv;
+                    float pz = sum2c(ti, nx[2], tj, ny[2], (float)cv);
                     w = w * ker[kb_index(pz - vk, fW, kerScale, kerCenter, K)];
                     long long lin = (long long)j1 * M + i1;
                     if (lin < M2 && lin >= 0)
@@ -212,6 +310,7 @@ void cpu_forward_project_f32(const float *vol, int V, const float *axes, int nAx
                              const float *ker, int K, float kerHWidth, int M,
                              float maskRadius, float *out, int nthreads)
 {
+    maskRadius = ref_mask_radius(maskRadius);
     const int cv = V / 2, ci = M / 2;
     const int convW = (int)roundf(kerHWidth);
     const float fW = (float)convW;
@@ -232,9 +331,9 @@ void cpu_forward_project_f32(const float *vol, int V, const float *axes, int nAx
                 float acc = 0.0f;
                 if (r <= maskRadius) {
                     float fi = (float)(i - ci), fj = (float)(j - ci);
-                    float x = nx[0] * fi + ny[0] * fj + (float)cv;
-                    float y = nx[1] * fi + ny[1] * fj + (float)cv;
-                    float z = nx[2] * fi + ny[2] * fj + (float)cv;
+                    float x = sum2c(nx[0], fi, ny[0], fj, (float)cv);
+                    float y = sum2c(nx[1], fi, ny[1], fj, (float)cv);
+                    float z = sum2c(nx[2], fi, ny[2], fj, (float)cv);
                     int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
                     for (int i1 = X - convW; i1 <= X + convW; i1++) {
                         float wi = ker[kb_index((float)i1 - x, fW, kerScale, kerCenter, K)];
@@ -260,6 +359,7 @@ void cpu_back_project_f32(float *volOut, int V, const float *axes, int nAxes,
                           const float *img, int M, float maskRadius,
                           const float *ker, int K, float kerHWidth, int nthreads)
 {
+    maskRadius = ref_mask_radius(maskRadius);
     const int cv = V / 2, ci = M / 2;
     const int convW = (int)roundf(kerHWidth);
     const float fW = (float)convW;
@@ -280,20 +380,22 @@ void cpu_back_project_f32(float *volOut, int V, const float *axes, int nAxes,
                 const float *nx = axes + 9 * a;
                 const float *ny = nx + 3;
                 const float *nz = ny + 3;
-                float fi = ((float)vi - cv) * nx[0] + ((float)vj - cv) * nx[1] + ((float)vk - cv) * nx[2] + ci;
-                float fj = ((float)vi - cv) * ny[0] + ((float)vj - cv) * ny[1] + ((float)vk - cv) * ny[2] + ci;
-                float fk = ((float)vi - cv) * nz[0] + ((float)vj - cv) * nz[1] + ((float)vk - cv) * nz[2];
+                const float di = (float)vi - cv, dj = (float)vj - cv, dk = (float)vk - cv;
+                float fi = sum3c(di, nx[0], dj, nx[1], dk, nx[2], (float)ci, 1);
+                float fj = sum3c(di, ny[0], dj, ny[1], dk, ny[2], (float)ci, 1);
+                float fk = sum3c(di, nz[0], dj, nz[1], dk, nz[2], 0.f, 0);
                 if (!(fabsf(fk) <= kerHWidth)) continue;
                 int I = (int)roundf(fi), J = (int)roundf(fj);
                 const float *im = img + (long long)a * M2;
                 float cum = 0.0f;
                 for (int j1 = J - convW; j1 <= J + convW; j1++) {
                     for (int i1 = I - convW; i1 <= I + convW; i1++) {
-                        float px = (i1 - ci) * nx[0] + (j1 - ci) * ny[0] + cv;
+                        const float ti = (float)(i1 - ci), tj = (float)(j1 - ci);
+                        float px = sum2c(ti, nx[0], tj, ny[0], (float)cv);
                         float w = ker[kb_index(px - vi, fW, kerScale, kerCenter, K)];
-                        float py = (i1 - ci) * nx[1] + (j1 - ci) * ny[1] + cv;
+                        float py = sum2c(ti, nx[1], tj, ny[1], (float)cv);
                         w = w * ker[kb_index(py - vj, fW, kerScale, kerCenter, K)];
-                        float pz = (i1 - ci) * nx[2] + (j1 - ci) * ny[2] + cv;
+                        float pz = sum2c(ti, nx[2], tj, ny[2], (float)cv);
                         w = w * ker[kb_index(pz - vk, fW, kerScale, kerCenter, K)];
                         long long lin = (long long)j1 * M + i1;
                         if (lin < M2 && lin >= 0)

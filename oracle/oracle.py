@@ -63,11 +63,52 @@ def _lib():
         lib.cpu_forward_project_f32.argtypes = [f32p, ci, f32p, ci, f32p, ci, cf, ci, cf, f32p, ci]
         lib.cpu_back_project_f32.argtypes = [f32p, ci, f32p, ci, f32p, ci, cf, f32p, ci, cf, ci]
         lib.cpu_max_threads.restype = ci
+        lib.orc_set_reading.argtypes = [ci]
+        lib.orc_set_reading.restype = ci
+        lib.orc_get_reading.restype = ci
         for fn in ("orc_forward_project", "orc_back_project", "orc_back_project_adjoint",
                    "cpu_forward_project_f32", "cpu_back_project_f32"):
             getattr(lib, fn).restype = None
         _LIB = lib
     return _LIB
+
+
+# Arithmetic readings of the reference kernels' decision expressions (the
+# reference is built by nvcc with no flags, src/CMakeLists.txt:80-83, and
+# nvcc's default --fmad=true contracts multiply-adds).  gridder_oracle.c
+# documents each; READING_FIRST is the default and the one the HIP kernels
+# follow.
+READING_SOURCE, READING_FIRST, READING_SECOND, READING_REASSOC, READING_INDEX = range(5)
+READINGS = {
+    "source": READING_SOURCE,
+    "first": READING_FIRST,
+    "second": READING_SECOND,
+    "reassoc": READING_REASSOC,
+    "index": READING_INDEX,
+}
+
+
+class reading:
+    """Context manager selecting the oracle's arithmetic reading:
+    ``with reading("source"): ...``.  Process-wide (not thread-safe)."""
+
+    def __init__(self, name):
+        self.value = READINGS[name] if isinstance(name, str) else int(name)
+
+    def __enter__(self):
+        prev = _lib().orc_set_reading(self.value)
+        if prev < 0:
+            raise ValueError(f"unknown reading {self.value}")
+        self.prev = prev
+        return self
+
+    def __exit__(self, *exc):
+        _lib().orc_set_reading(self.prev)
+        return False
+
+
+def current_reading() -> int:
+    return int(_lib().orc_get_reading())
 
 
 # ----------------------------------------------------------------------------

@@ -5,7 +5,7 @@ import ctypes
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import assert_parity, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -101,8 +101,7 @@ def test_reconstruct_object(cuda, oracle, ker32):
     cas_imgs = oracle.imgs_to_cas_imgs(imgs, m).astype(np.float32)
     bp = oracle.back_project(cas_imgs, axes, ker32, v)
     dens = oracle.back_project(np.ones_like(cas_imgs), axes, ker32, v)
-    ref = oracle.reconstruct(bp, dens, n)
-    assert rel_l2(rec, ref) < 1e-4  # the 1e-6 in the density division amplifies FP32 noise where density ~ 0
+    _check_reconstruction(oracle, cuda, rec, bp, dens, cas_imgs, axes, ker32, n, "n=16 grid")
     # the reference's own acceptance check (BackProjectTests.m:57-66): max |recon - vol| <= 1
     assert np.abs(rec - vol).max() <= 1.0 + np.abs(vol).max()
     g.delete()
@@ -225,8 +224,44 @@ def test_config_c2_n128_grid_subset(cuda, oracle, ker32):
     cas_vol = g.casVolumeToHost()
     cas_imgs = oracle.imgs_to_cas_imgs(imgs, m).astype(np.float32)
     bp = oracle.back_project(cas_imgs, axes, ker32, v)
-    assert rel_l2(cas_vol, bp) < TOL
+    # relative L2 and the max-abs guard: a flipped table index or rounded
+    # position in any voxel (the fused form A runs in every brick here) fails
+    assert_parity(cas_vol, bp)
     rec = g.reconstructVol(imgs, axes)
     dens = oracle.back_project(np.ones_like(cas_imgs), axes, ker32, v)
-    assert rel_l2(rec, oracle.reconstruct(bp, dens, n)) < 1e-4
+    _check_reconstruction(oracle, cuda, rec, bp, dens, cas_imgs, axes, ker32, n, "C2 subset")
     g.delete()
+
+
+def _check_reconstruction(oracle, cuda, rec, bp, dens, cas_imgs, axes, ker32, n, tag):
+    """reconstructVol = CASToVol(bp / (density + 1e-6)) (MultiGPUGridder_Matlab_Class.m:
+    284-312).  Where the plane density is well above zero the division is
+    well conditioned and the quotient meets the 1e-5 bar; where it is ~0 the
+    1e-6 amplifies FP32 accumulation noise in bp by up to 1e6, and the inverse
+    FFT spreads that over the whole volume.  Checked: (1) the device quotient
+    against the oracle's on voxels with density > 1e-3 of its peak, at 1e-5;
+    (2) that volume rebuilt with the oracle's quotient on the remaining voxels,
+    at 1e-5, which shows the full-volume error comes from those voxels alone;
+    (3) the full reconstruction, at 1e-4, with its error printed."""
+    import torch
+
+    from multigpugridder_amd import ops
+
+    m, v = 2 * n, 2 * n + 6
+    q_ref = bp / (dens + 1e-6)
+    num = torch.zeros((v, v, v), device=cuda)
+    den = torch.zeros((v, v, v), device=cuda)
+    ops.back_project(torch.from_numpy(cas_imgs).to(cuda), torch.from_numpy(axes).to(cuda), v,
+                     ker=torch.from_numpy(ker32).to(cuda), out=num, density=den)
+    q_gpu = ops.divide_density(num, den).cpu().numpy().astype(np.float64)
+    good = dens > 1e-3 * dens.max()
+    e_good = rel_l2(q_gpu[good], q_ref[good])
+    rec_ref = oracle.cas_to_vol(q_ref, n)
+    e_mix = rel_l2(oracle.cas_to_vol(np.where(good, q_gpu, q_ref), n), rec_ref)
+    e_full = rel_l2(rec, rec_ref)
+    print(f"{tag} reconstructVol: quotient on density > 1e-3 peak ({good.mean():.1%} of voxels) "
+          f"{e_good:.2e}; volume with those voxels from the device {e_mix:.2e}; full volume {e_full:.2e}",
+          flush=True)
+    assert e_good < TOL
+    assert e_mix < TOL
+    assert e_full < 1e-4

@@ -271,6 +271,69 @@ def test_back_project_adjoint(cuda, oracle, ker32, n, p):
     assert_parity(_np(out), oracle.back_project_adjoint(y, axes, ker32, v))
 
 
+def test_kernels_follow_the_contracted_reading(cuda, oracle, ker32):
+    """The kernels take their rounding decisions as the reference compiled by
+    nvcc with its default --fmad=true does (the reading the oracle pins,
+    "first").  Readings that differ only in how multiply-adds are contracted
+    give results the parity bar tells apart: the back projection of noise
+    images moves by > 1e-5 under the unfused source order, and the forward
+    projection by > 2e-6 of the peak under nested-FMA reassociation."""
+    import torch
+
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.synth import blob_volume, random_directions
+
+    n, p = 32, 60
+    m, v = 2 * n, 2 * n + 6
+    print(f"reading discrimination n={n} p={p}", flush=True)
+    cas = oracle.vol_to_cas(blob_volume(n, 1), 2, 3).astype(np.float32)
+    axes = random_directions(p, 7)
+    y = np.random.default_rng(3).standard_normal((p, m, m)).astype(np.float32)
+    fp = _np(ops.forward_project(_t(cas, cuda), _t(axes, cuda), m, ker=_t(ker32, cuda)))
+    out = torch.zeros((v, v, v), device=cuda)
+    ops.back_project(_t(y, cuda), _t(axes, cuda), v, ker=_t(ker32, cuda), out=out)
+    bp = _np(out)
+    assert_parity(fp, oracle.forward_project(cas, axes, ker32, m))
+    assert_parity(bp, oracle.back_project(y, axes, ker32, v))
+    with oracle.reading("source"):
+        bp_src = oracle.back_project(y, axes, ker32, v)
+    with oracle.reading("reassoc"):
+        fp_re = oracle.forward_project(cas, axes, ker32, m)
+    print(f"  BP vs source reading {rel_l2(bp, bp_src):.2e}; FP vs reassoc max/peak "
+          f"{np.abs(fp - fp_re).max() / np.abs(fp_re).max():.2e}", flush=True)
+    assert rel_l2(bp, bp_src) > 1e-5
+    assert np.abs(fp - fp_re).max() > 2e-6 * np.abs(fp_re).max()
+
+
+def test_fractional_mask_radius_is_truncated(cuda, oracle, ker32):
+    """RunKernel takes the mask radius as an int (gpuForwardProject.h:57,
+    gpuBackProject.h:56): a radius of 19.5 behaves exactly as 19."""
+    import torch
+
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.synth import blob_volume, random_directions
+
+    n, p = 24, 12
+    m, v = 2 * n, 2 * n + 6
+    print("fractional mask radius 19.5", flush=True)
+    cas = oracle.vol_to_cas(blob_volume(n, 2), 2, 3).astype(np.float32)
+    axes = random_directions(p, 12)
+    y = np.random.default_rng(12).standard_normal((p, m, m)).astype(np.float32)
+    dc, da, dy, dk = _t(cas, cuda), _t(axes, cuda), _t(y, cuda), _t(ker32, cuda)
+    for r in (19.5, 19.0):
+        fp = _np(ops.forward_project(dc, da, m, mask_radius=r, ker=dk))
+        assert_parity(fp, oracle.forward_project(cas, axes, ker32, m, mask_radius=19.0))
+        out = torch.zeros((v, v, v), device=cuda)
+        ops.back_project(dy, da, v, mask_radius=r, ker=dk, out=out)
+        assert_parity(_np(out), oracle.back_project(y, axes, ker32, v, mask_radius=19.0))
+        adj = torch.zeros((v, v, v), device=cuda)
+        ops.back_project_adjoint(dy, da, v, mask_radius=r, ker=dk, out=adj)
+        assert_parity(_np(adj), oracle.back_project_adjoint(y, axes, ker32, v, mask_radius=19.0))
+    f195 = ops.forward_project(dc, da, m, mask_radius=19.5, ker=dk)
+    assert torch.equal(f195, ops.forward_project(dc, da, m, mask_radius=19.0, ker=dk))
+    assert not torch.equal(f195, ops.forward_project(dc, da, m, mask_radius=20.0, ker=dk))
+
+
 # ---------------------------------------------------------------------------
 # Image and volume chains (rocFFT + fused pre/post kernels)
 # ---------------------------------------------------------------------------

@@ -1,14 +1,15 @@
-"""The back-projection kernel replaces the reference's table-index arithmetic
-(r = fl(p - v), clamp, fl(fl(r*125) + 250), roundf; Projection/
-gpuBackProject.cu:77-95) by two fused forms that each drop operations from
-every tap.  They must pick the same table entry for every float tap position
-p within 4.5 of the voxel coordinate v.  This runs `index_forms.c` over every
-such float for voxel coordinates near each range boundary and at large
-volumes (the kernel's own exhaustive run covered every v up to 2,060).
-
-Form B (v >= 9) must agree everywhere; form A (v >= 130) must agree above
-its bound and must NOT agree below it (a double rounding it skips matters
-there), which shows the check can fail."""
+"""The projection kernels replace the reference's table-index arithmetic, as
+nvcc contracts it (r = fl(p - v), clamp to [-2, 2], roundf(fma(r, 125, 250));
+Projection/gpuBackProject.cu:78-81, gpuForwardProject.cu:65-68), by
+floor(s + 0.5) (one v_cvt_rpi_i32_f32) on an unclamped r with a saturated
+index (form N), and the back projector by a single FMA of the tap position
+itself (form A, s = fma(p, 125, 250 - 125 v), for voxel coordinates v >= 3).
+Both must pick the same table entry as the reference.  This runs
+`index_forms.c`: form N over every float r in [-4.5, 4.5], form A over every
+float tap position p within 4.5 of v for coordinates around its bound and at
+large volumes, each with floor(s + 0.5) exact and with s + 0.5 rounded to
+FP32.  Form A must also fail at v = 1 (p - v is inexact for p < 1), which
+shows the check can fail."""
 import os
 import subprocess
 
@@ -20,8 +21,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 @pytest.fixture(scope="module")
 def checker(tmp_path_factory):
     exe = str(tmp_path_factory.mktemp("idx") / "index_forms")
-    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-o", exe, os.path.join(HERE, "index_forms.c"), "-lm"],
-                   check=True)
+    subprocess.run(["gcc", "-O2", "-ffp-contract=off", "-march=x86-64-v3", "-o", exe,
+                    os.path.join(HERE, "index_forms.c"), "-lm"], check=True)
     return exe
 
 
@@ -29,26 +30,28 @@ def run(checker, *ranges):
     out = subprocess.run([checker, *ranges], check=True, capture_output=True, text=True).stdout
     rows = {}
     for line in out.strip().splitlines():
-        lo, hi, n, ma, mb = (int(x) for x in line.split())
-        rows[(lo, hi)] = (n, ma, mb)
+        f = line.split()
+        if f[0] == "r":
+            rows["r"] = tuple(int(x) for x in f[1:])
+        else:
+            lo, hi, n, me, mf = (int(x) for x in f)
+            rows[(lo, hi)] = (n, me, mf)
     return rows
 
 
-def test_form_b_matches_reference_index(checker):
-    rows = run(checker, "9:14", "120:140", "255:262", "515:520", "1027:1030")
-    for (lo, hi), (n, _, mb) in rows.items():
-        assert n > 0
-        assert mb == 0, f"form B differs for v in [{lo}, {hi}]"
+def test_form_n_matches_reference_index_for_every_offset(checker):
+    n, me, mf = run(checker, "r")["r"]
+    assert n > 2_000_000_000
+    assert me == 0 and mf == 0
 
 
-def test_form_a_matches_above_its_bound(checker):
-    rows = run(checker, "130:140", "255:262", "515:520", "1027:1030")
-    for (lo, hi), (n, ma, _) in rows.items():
+def test_form_a_matches_reference_index(checker):
+    rows = run(checker, "4:14", "120:140", "255:262", "515:520", "1027:1030", "2050:2060")
+    for (lo, hi), (n, me, mf) in rows.items():
         assert n > 0
-        assert ma == 0, f"form A differs for v in [{lo}, {hi}]"
+        assert me == 0 and mf == 0, f"form A differs for v in [{lo}, {hi}]"
 
 
 def test_form_a_fails_below_its_bound(checker):
-    rows = run(checker, "100:129")
-    n, ma, mb = rows[(100, 129)]
-    assert ma > 0 and mb == 0
+    n, me, mf = run(checker, "1:1")[(1, 1)]
+    assert me > 0 and mf > 0
