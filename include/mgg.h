@@ -173,6 +173,17 @@ int mgg_op_back_project_adjoint(float *cas_volume, int v, const float *axes, int
  * Uses a batched rocFFT C2R; cas_images is clobbered (used as scratch). */
 int mgg_op_cas_images_to_images(float *cas_images, float *images, int n, int m, int nn,
                                 void *stream);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mgg_op_cas_images_to_images without any allocation and
+ * without clobbering cas_images.  Only the fused chains' sizes are accepted
+ * (m = 128, 192, 256, 384 or 512, nn <= m, m - nn even; others return
+ * MGG_ERR_UNSUPPORTED); the intermediate lives in a fixed 32 MiB device array
+ * per GPU, held from library load to process end, and n is processed in chunks
+ * that fit it.  Calls that overlap on one GPU share that array and race:
+ * order them on one stream. */
+int mgg_op_cas_images_to_images_noalloc(const float *cas_images, float *images, int n, int m, int nn,
+                                        void *stream);
 /* Gridders/gpuGridder.cpp:520-596: n images (nn x nn) -> CAS images (m x m). */
 int mgg_op_images_to_cas_images(const float *images, float *cas_images, int n, int m, int nn,
                                 void *stream);

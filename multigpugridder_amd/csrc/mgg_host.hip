@@ -768,6 +768,75 @@ int mgg_op_cas_images_to_images(float *cas_images, float *images, int n, int m, 
     });
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free CAS images -> images.  For callers that cannot allocate
+// device memory (code under HIP graph capture, serving loops, harnesses that
+// forbid hipMalloc).  The fused chains' half-spectrum intermediate lives in a
+// fixed device array, `scratch`, of kNoAllocScratchBytes per GPU (every GPU
+// the library runs on holds it until the process ends); its device address is
+// taken once per device with hipGetSymbolAddress and only that address is
+// handed to kernels.  Every call on one GPU shares the array: calls that
+// overlap on one GPU (different streams or threads) race.  Each chunk's
+// intermediate is written whole by the first kernel before the second reads
+// it, so nothing needs clearing between calls.
+// ---------------------------------------------------------------------------
+namespace {
+constexpr size_t kNoAllocScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) float2 scratch[kNoAllocScratchBytes / sizeof(float2)];
+
+constexpr int kNoAllocMaxDevices = 64;
+std::mutex g_noalloc_mu;
+float2 *g_noalloc_ptr[kNoAllocMaxDevices] = {};
+
+// Device address of this GPU's `scratch` (the current device must be the one
+// the caller's stream belongs to).
+float2 *noalloc_scratch()
+{
+    const int dev = current_device();
+    if (dev < 0 || dev >= kNoAllocMaxDevices)
+        raise(MGG_ERR_UNSUPPORTED, "device %d beyond the %d devices the allocation-free path tracks", dev,
+              kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            raise(MGG_ERR_HIP, "hipGetSymbolAddress(scratch) failed on device %d: %s", dev, hipGetErrorString(e));
+        g_noalloc_ptr[dev] = static_cast<float2 *>(ptr);
+    }
+    return g_noalloc_ptr[dev];
+}
+}  // namespace
+
+int mgg_op_cas_images_to_images_noalloc(const float *cas_images, float *images, int n, int m, int nn,
+                                        void *stream)
+{
+    return guarded([&] {
+        if (n == 0) return;
+        need(cas_images, "cas_images"); need(images, "images");
+        if (n < 0 || m <= 0 || nn <= 0 || nn > m || (m % 2) || ((m - nn) % 2))
+            raise(MGG_ERR_INVALID_ARGUMENT, "bad sizes n=%d m=%d nn=%d", n, m, nn);
+        if (!fused_chain_supported(m, nn))
+            raise(MGG_ERR_UNSUPPORTED,
+                  "the allocation-free image chain runs the fused chains only (m = 128, 192, 256, 384, 512); "
+                  "m = %d needs rocFFT plans, which allocate", m);
+        const size_t perImage = sizeof(float2) * (size_t)nn * (size_t)(m / 2 + 1);
+        if (perImage > kNoAllocScratchBytes)
+            raise(MGG_ERR_INVALID_ARGUMENT, "one image needs %zu scratch bytes, the fixed scratch holds %zu",
+                  perImage, kNoAllocScratchBytes);
+        const int chunk = (int)std::min<size_t>(kNoAllocScratchBytes / perImage, 65535);
+        float2 *tmp = noalloc_scratch();
+        hipStream_t s = as_stream(stream);
+        const float scale = 1.0f / ((float)m * (float)m);
+        for (int a = 0; a < n; a += chunk) {
+            const int b = std::min(chunk, n - a);
+            check_launch(launch_fused_cas_to_images(cas_images + (size_t)a * m * m, tmp, images + (size_t)a * nn * nn,
+                                                    m, nn, b, scale, s),
+                         "fused_cas_to_images (allocation-free)");
+        }
+    });
+}
+
 int mgg_op_images_to_cas_images(const float *images, float *cas_images, int n, int m, int nn, void *stream)
 {
     return guarded([&] {
