@@ -74,9 +74,10 @@ def profile_value(path: str, key: str):
 
 
 def run_peak_probe():
-    """Measured VALU / random-LDS / copy peaks of this GPU (multigpugridder_amd/peak_probe,
-    built with the library); None when the probe is not built or fails."""
-    exe = os.path.join(ROOT, "multigpugridder_amd", "peak_probe")
+    """Measured VALU / random-LDS / copy peaks of this GPU (``peak_probe``, built
+    from ``peak_probe.hip`` beside this file by __graft_entry__.build(); not part of
+    the library); None when the probe is not built or fails."""
+    exe = os.path.join(ROOT, "peak_probe")
     if not os.path.exists(exe):
         return None
     try:
@@ -89,34 +90,66 @@ def run_peak_probe():
     return json.loads(lines[-1]) if lines else None
 
 
-def cpu_baseline(n: int, p_sample: int, seed: int = 1):
-    """Time the oracle's FP32 OpenMP path (the reference kernels' loop
-    structure) on p_sample directions of the same geometry."""
-    from oracle import oracle as orc
-
+def _cpu_path_times(orc, n, axes, reps, seed=1):
+    """Median over ``reps`` runs (after one warm-up) of the CPU path's stages on
+    one workload: the two gathers (the oracle's FP32 OpenMP restatement of the
+    reference kernels) and the image chains (the host FFT path of
+    RunFFTOnGPU=0, numpy)."""
+    from multigpugridder_amd import hostmath
     from multigpugridder_amd.synth import blob_volume
 
     m = 2 * n
     v = m + 6
     ker = orc.kb_table().astype(np.float32)
     cas = orc.vol_to_cas(blob_volume(n, seed), 2, 3).astype(np.float32)
-    axes = orc.uniform_axes(100, 100)[:: max(1, 10000 // p_sample)][:p_sample]
+    t = {"fp": [], "fp_chain": [], "bp_chain": [], "bp": []}
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        ci = orc.cpu_forward_project_f32(cas, axes, ker, m)
+        t1 = time.perf_counter()
+        imgs = hostmath.imgs_from_cas_imgs(ci, n)
+        t2 = time.perf_counter()
+        ci2 = hostmath.cas_imgs_from_imgs(imgs, m)
+        t3 = time.perf_counter()
+        orc.cpu_back_project_f32(np.ascontiguousarray(ci2, dtype=np.float32), axes, ker, v)
+        t4 = time.perf_counter()
+        if r:  # the first run is the warm-up
+            for k, dt in zip(t, (t1 - t0, t2 - t1, t3 - t2, t4 - t3)):
+                t[k].append(dt)
+    return {k: float(np.median(x)) for k, x in t.items()}
+
+
+def cpu_baseline(n: int, p_sample: int, reps: int = 5):
+    """The CPU path timed beside the GPU: the reference kernels' loop structure
+    in FP32 with OpenMP over all host threads, plus the host image chains, on
+    a bounded sample of the headline workload (``p_sample`` evenly spaced
+    grid directions at N=``n``) and on the CPU-runnable configuration (N=64,
+    500 random directions, seed 100).  Median of ``reps`` runs each."""
+    from oracle import oracle as orc
+
+    from multigpugridder_amd.synth import random_directions
+
     threads = orc.cpu_threads()
-    orc.cpu_forward_project_f32(cas, axes[:2], ker, m)  # warm-up
-    t0 = time.perf_counter()
-    ci = orc.cpu_forward_project_f32(cas, axes, ker, m)
-    t1 = time.perf_counter()
-    orc.cpu_back_project_f32(ci, axes, ker, v)
-    t2 = time.perf_counter()
+    axes = orc.uniform_axes(100, 100)[:: max(1, 10000 // p_sample)][:p_sample]
+    big = _cpu_path_times(orc, n, axes, reps)
+    c1 = _cpu_path_times(orc, 64, random_directions(500, 100), reps)
+    tot = sum(big.values())
+
+    def rates(t, p):
+        return {f"{k}_dirs_per_s": round(p / v, 1) for k, v in t.items()} | {
+            "fwd_back_dirs_per_s": round(p / sum(t.values()), 1)}
+
     return {
-        "value": p_sample / (t2 - t0),
-        "unit": "projections/s (fwd+back pairs, CAS gather kernels only)",
+        "value": round(p_sample / tot, 2),
+        "unit": "projections/s (fwd+back pairs: gathers and image chains)",
         "cores": threads,
         "kind": "port",
-        "sample": f"N={n} (V={v}, M={m}), {p_sample} of the 10,000 grid directions: "
-                  f"forward {t1 - t0:.2f}s, back {t2 - t1:.2f}s, FP32 OpenMP",
-        "fwd_per_s": p_sample / (t1 - t0),
-        "back_per_s": p_sample / (t2 - t1),
+        "sample": f"N={n} (V={n * 2 + 6}, M={n * 2}), {p_sample} of the 10,000 grid directions, "
+                  f"median of {reps}: forward gather {big['fp']:.2f}s, inverse chain {big['fp_chain']:.2f}s, "
+                  f"forward chain {big['bp_chain']:.2f}s, back gather {big['bp']:.2f}s; gathers FP32 OpenMP "
+                  f"on {threads} threads, chains numpy (host FFT path)",
+        "stages": rates(big, p_sample),
+        "c1_n64_500_random": rates(c1, 500) | {"median_s": {k: round(v, 3) for k, v in c1.items()}},
     }
 
 
@@ -300,6 +333,132 @@ def serving_bench(sizes, n_req, lambdas, seed=100, budget_gb=64, quiet=False):
     return res
 
 
+def _events(torch):
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def reconstruct_bench(dev, imgs, axes, ker, n, chunk, reps=3):
+    """reconstructVol on the headline workload, device-resident
+    (MultiGPUGridder_Matlab_Class.m:284-312): images -> CAS images, back
+    projection with the plane density accumulated in the same pass, the FP64
+    density division and CAS volume -> volume.  Median of ``reps``."""
+    import torch
+
+    from multigpugridder_amd import ops
+
+    m, v = 2 * n, 2 * n + 6
+    p = imgs.shape[0]
+    buf = torch.empty((chunk, m, m), device=dev)
+    num = torch.empty((v, v, v), device=dev)
+    den = torch.empty((v, v, v), device=dev)
+    times = []
+    for r in range(reps + 1):
+        a, b = _events(torch)
+        a.record()
+        num.zero_()
+        den.zero_()
+        for s0 in range(0, p, chunk):
+            c = min(chunk, p - s0)
+            ops.images_to_cas_images(imgs[s0:s0 + c], m, out=buf[:c])
+            ops.back_project(buf[:c], axes[s0:s0 + c], v, ker=ker, out=num, density=den)
+        vol = ops.cas_volume_to_volume(ops.divide_density(num, den), n)
+        b.record()
+        torch.cuda.synchronize()
+        if r:
+            times.append(a.elapsed_time(b))
+    del vol
+    ms = float(np.median(times))
+    return {"ms": round(ms, 3), "directions": p, "dirs_per_s": round(p / (ms * 1e-3), 1),
+            "what": "images->CAS images, back projection with plane density in one pass, FP64 division, "
+                    "CAS volume->volume; device-resident, median of 3"}
+
+
+def dropin_bench(n, axes_np, reps=2):
+    """The plain drop-in call with host arrays (the Matlab class path through
+    the C ABI): forwardProject of every direction to host images, then
+    backProject of those images and getVol.  Includes every host<->device
+    copy (PCIe), so it is reported beside ``value``, never as it."""
+    from multigpugridder_amd import MultiGPUGridder
+    from multigpugridder_amd.synth import blob_volume
+
+    p = axes_np.shape[0]
+    vol = blob_volume(n, 1)
+    g = MultiGPUGridder(n, p, 2)
+    g.setVolume(vol)
+    tf, tb = [], []
+    for r in range(reps + 1):
+        t0 = time.perf_counter()
+        im = g.forwardProject(axes_np)
+        t1 = time.perf_counter()
+        g.resetVolume()
+        g.backProject(im, axes_np)
+        g.getVol()
+        t2 = time.perf_counter()
+        if r:
+            tf.append(t1 - t0)
+            tb.append(t2 - t1)
+    g.delete()
+    f, b = float(np.median(tf)), float(np.median(tb))
+    return {"fwd_dirs_per_s": round(p / f, 1), "back_dirs_per_s": round(p / b, 1),
+            "fwd_back_dirs_per_s": round(p / (f + b), 1), "fwd_s": round(f, 3), "back_s": round(b, 3),
+            "what": f"MultiGPUGridder(N={n}).forwardProject / resetVolume + backProject + getVol on "
+                    f"{p} grid directions, host arrays (pinned staging, PCIe included), median of {reps}"}
+
+
+def differentiation_bench(dev, n=96, p=2048, reps=5):
+    """The differentiation configuration: loss ||P v - y||^2 at N=96, interp 2,
+    2,048 random directions, P the full spatial projector (Vol->CAS, gather,
+    image chain) and its backward the exact transposes.  Median ms of the
+    forward (projection and loss) and of loss.backward(), and of the two
+    gather kernels alone on the same CAS images."""
+    import torch
+
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.autograd import project
+    from multigpugridder_amd.synth import blob_volume, random_directions
+
+    axes = torch.from_numpy(random_directions(p, 104)).to(dev)
+    y = project(torch.from_numpy(blob_volume(n, 2)).to(dev), axes)
+    g = torch.Generator(device="cpu").manual_seed(3)
+    y = (y + 0.1 * float(y.std()) * torch.randn(y.shape, generator=g).to(dev)).contiguous()
+    vol = torch.from_numpy(blob_volume(n, 1)).to(dev).requires_grad_(True)
+    fwd, bwd = [], []
+    for r in range(reps + 1):
+        a, b = _events(torch)
+        c = torch.cuda.Event(enable_timing=True)
+        a.record()
+        loss = ((project(vol, axes) - y) ** 2).sum()
+        b.record()
+        loss.backward()
+        c.record()
+        torch.cuda.synchronize()
+        vol.grad = None
+        if r:
+            fwd.append(a.elapsed_time(b))
+            bwd.append(b.elapsed_time(c))
+    m, v = 2 * n, 2 * n + 6
+    cas = ops.volume_to_cas_volume(vol.detach())
+    ci = ops.forward_project(cas, axes, m)
+    acc = torch.zeros((v, v, v), device=dev)
+    kt = {"fp": [], "adj": []}
+    for r in range(reps + 1):
+        a, b = _events(torch)
+        c = torch.cuda.Event(enable_timing=True)
+        a.record()
+        ops.forward_project(cas, axes, m, out=ci)
+        b.record()
+        ops.back_project_adjoint(ci, axes, v, out=acc)
+        c.record()
+        torch.cuda.synchronize()
+        if r:
+            kt["fp"].append(a.elapsed_time(b))
+            kt["adj"].append(b.elapsed_time(c))
+    return {"config": f"N={n} interp 2, {p} random directions (seed 104), loss ||P v - y||^2",
+            "forward_ms": round(float(np.median(fwd)), 3), "backward_ms": round(float(np.median(bwd)), 3),
+            "gather_kernel_ms": round(float(np.median(kt["fp"])), 3),
+            "adjoint_kernel_ms": round(float(np.median(kt["adj"])), 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,8 +470,12 @@ def main():
     ap.add_argument("--chunk", type=int, default=2000, help="directions per launch")
     ap.add_argument("--streams", type=int, default=2,
                     help="streams the chunks of a step are spread over (image chains overlap projectors)")
-    ap.add_argument("--cpu-sample", type=int, default=2000,
-                    help="grid directions the CPU baseline runs (about 15 s of CPU work on 16 threads)")
+    ap.add_argument("--cpu-sample", type=int, default=400,
+                    help="grid directions the CPU baseline runs at N=128 (median of --cpu-reps runs; "
+                         "about 25 s of CPU work on 16 threads with the N=64 configuration)")
+    ap.add_argument("--cpu-reps", type=int, default=5)
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the reconstructVol, host drop-in and differentiation measurements")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this many MiB (a known-size "
@@ -329,6 +492,8 @@ def main():
     ap.add_argument("--serve-sizes", default="128",
                     help="volume sizes of the resident volumes (one volume each), comma separated")
     ap.add_argument("--serve-only", action="store_true", help="run only the serving leg, print its JSON")
+    ap.add_argument("--diff-only", action="store_true",
+                    help="run only the differentiation measurement (N=96, 2,048 directions), print its JSON")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -349,6 +514,9 @@ def main():
         return
 
     dev = torch.device("cuda", 0)
+    if args.diff_only:
+        print(json.dumps(differentiation_bench(dev)), flush=True)
+        return
     n = args.n
     m = 2 * n
     v = m + 6
@@ -520,14 +688,19 @@ def main():
             "kernels": kernels, "probe": probe,
             "source": "VALU instructions per direction: rocprofv3 SQ_INSTS_VALU (profiles/issue_kernels_n128.md); "
                       "peak: peak_probe on this GPU (independent FP32 adds)"}
+    if not args.no_extras:
+        out["reconstruct"] = reconstruct_bench(dev, imgs, axes, ker, n, chunk)
+    del cas_bufs, imgs, bp_vol
+    torch.cuda.empty_cache()
+    if not args.no_extras:
+        out["dropin"] = dropin_bench(n, axes_np)
+        out["differentiation"] = differentiation_bench(dev)
     if args.serve_requests > 0:
-        del cas_bufs, imgs, bp_vol
-        torch.cuda.empty_cache()
         sizes = [int(x) for x in args.serve_sizes.split(",")]
         lams = [float(x) for x in args.serve_lambdas.split(",") if x]
         out["serving"] = serving_bench(sizes, args.serve_requests, lams)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n, args.cpu_sample)
+        out["cpu_baseline"] = cpu_baseline(n, args.cpu_sample, args.cpu_reps)
     print(json.dumps(out), flush=True)
 
 

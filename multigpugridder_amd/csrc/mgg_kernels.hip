@@ -686,6 +686,209 @@ __global__ __launch_bounds__(kThreads) void adj_kernel(FPArgs p, const float *__
 }
 
 // ---------------------------------------------------------------------------
+// Exact transpose, LDS-box scatter (kerHWidth 2, tables of at most FPKER
+// entries).  A wave owns one 8x8 pixel tile of one direction, as in
+// fp_lds_tile: it bounds the voxel box its pixels' 5x5x5 stencils touch (the
+// same corner bound), zeroes that box in LDS, adds every tap's w * value into
+// it (ordered plain read-modify-write in rounds that keep colliding lanes
+// apart; LDS float atomics for frames that are not orthonormal), and adds each touched
+// box voxel to the volume with one global atomic: about 11 global atomics per
+// pixel at N=128 instead of one per tap (125).  Box voxels map to the volume through the reference's
+// linear index (gpuForwardProject.cu:90), so the transpose keeps the forward
+// gather's row wrap.  A tile whose box exceeds the LDS budget runs as two 8x4
+// halves; a half that still does not fit scatters with global atomics.
+// ---------------------------------------------------------------------------
+template <bool FAST>
+__device__ __forceinline__ void adj_lds_tile(const FPArgs &p, const float *__restrict__ sker,
+                                             float *__restrict__ box, const float *__restrict__ img,
+                                             float *__restrict__ volAcc, unsigned work, int ln)
+{
+    const int a = (int)(work / (unsigned)p.tilesPerImg);
+    const int tile = (int)(work - (unsigned)a * (unsigned)p.tilesPerImg);
+    const int ty = tile / p.tilesPerRow, tx = tile - ty * p.tilesPerRow;
+    const int i0 = tx * FPT, j0 = ty * FPT;
+    const int i = i0 + (ln & 7), j = j0 + (ln >> 3);
+    const int M = p.M, V = p.V, ci = M / 2, cv = V / 2;
+    const bool pix = i < M && j < M;
+    const bool inMask = pix && (i - ci) * (i - ci) + (j - ci) * (j - ci) <= p.maskD2;
+    const float val = inMask ? img[(size_t)a * M * M + (size_t)j * M + i] : 0.f;
+    const bool act = inMask && val != 0.f;
+    if (__ballot(act) == 0ull) return;
+
+    const float *ax = p.axes + (size_t)9 * a;
+    const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
+    const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
+    const float fi = (float)(i - ci), fj = (float)(j - ci);
+    const float x = ref_sum2c(u0, fi, v0, fj, (float)cv);
+    const float y = ref_sum2c(u1, fi, v1, fj, (float)cv);
+    const float z = ref_sum2c(u2, fi, v2, fj, (float)cv);
+    const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
+    const long long V3 = (long long)V * V * V;
+    const long long VV = (long long)V * V;
+    // [u v] orthonormal to 1e-4 (wave-uniform: one direction per tile)
+    const bool orthoFrame = fabsf(u0 * u0 + u1 * u1 + u2 * u2 - 1.f) <= 1e-4f &&
+                            fabsf(v0 * v0 + v1 * v1 + v2 * v2 - 1.f) <= 1e-4f &&
+                            fabsf(u0 * v0 + u1 * v1 + u2 * v2) <= 1e-4f;
+
+    float wi[5], wj[5], wk[5];
+#pragma unroll
+    for (int t = 0; t < 5; t++) {
+        if (FAST) {
+            wi[t] = kb_fast_fp(p.ker, (float)(X - 2 + t) - x);
+            wj[t] = kb_fast_fp(p.ker, (float)(Y - 2 + t) - y);
+            wk[t] = kb_fast_fp(p.ker, (float)(Z - 2 + t) - z);
+        } else {
+            wi[t] = sker[kb_index((float)(X - 2 + t) - x, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wj[t] = sker[kb_index((float)(Y - 2 + t) - y, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wk[t] = sker[kb_index((float)(Z - 2 + t) - z, p.fW, p.kerScale, p.kerCenter, p.K)];
+        }
+    }
+
+    // The reference's bounds test on the linear index, one global atomic per
+    // tap (gpuForwardProject.cu:90).
+    auto scatterGlobal = [&]() {
+#pragma unroll
+        for (int cc = 0; cc < 5; cc++)
+#pragma unroll
+            for (int bb = 0; bb < 5; bb++)
+#pragma unroll
+                for (int aa = 0; aa < 5; aa++) {
+                    const long long lin = (long long)(Z - 2 + cc) * VV + (long long)(Y - 2 + bb) * V + (X - 2 + aa);
+                    const float w = wi[aa] * wj[bb] * wk[cc];
+                    if (lin < V3 && lin >= 0 && w != 0.f) unsafeAtomicAdd(volAcc + lin, w * val);
+                }
+    };
+
+    // Scatters the pixels of the tile rows j0 + rlo .. j0 + rhi (lanes with
+    // mine set) through an LDS box; false (nothing done) when the box does not
+    // fit the wave's budget.  Uniform across the wave.
+    auto scatter = [&](int rlo, int rhi, bool mine) -> bool {
+        const float ai = (float)(i0 - ci), bi = (float)(i0 + FPT - 1 - ci);
+        const float aj = (float)(j0 + rlo - ci), bj = (float)(j0 + rhi - ci);
+        auto lo = [&](float uu, float vv) {
+            return (int)floorf(fminf(uu * ai, uu * bi) + fminf(vv * aj, vv * bj) + (float)cv - 1e-3f) - 2;
+        };
+        auto hi = [&](float uu, float vv) {
+            return (int)ceilf(fmaxf(uu * ai, uu * bi) + fmaxf(vv * aj, vv * bj) + (float)cv + 1e-3f) + 2;
+        };
+        const int bx0 = __builtin_amdgcn_readfirstlane(lo(u0, v0));
+        const int by0 = __builtin_amdgcn_readfirstlane(lo(u1, v1));
+        const int bz0 = __builtin_amdgcn_readfirstlane(lo(u2, v2));
+        const int Dx = __builtin_amdgcn_readfirstlane(hi(u0, v0)) - bx0 + 1;
+        const int Dy = __builtin_amdgcn_readfirstlane(hi(u1, v1)) - by0 + 1;
+        const int Dz = __builtin_amdgcn_readfirstlane(hi(u2, v2)) - bz0 + 1;
+        if (!(Dx >= 5 && Dy >= 5 && Dz >= 5 && Dx <= 64 && Dy <= 64 && Dz <= 64)) return false;
+        const int RP = Dx, PP = Dx * Dy, total = PP * Dz;
+        if (total > FPBOX) return false;
+        for (int e = ln; e < total; e += 64) box[e] = 0.f;
+        wave_lds_sync_fp();
+        // Every finite in-row pixel's stencil lies inside the box (the corner
+        // bound); a lane whose does not (non-finite directions) scatters
+        // through the checked global path instead of outside its box.
+        const bool inBox = (unsigned)(X - 2 - bx0) <= (unsigned)(Dx - 5) &&
+                           (unsigned)(Y - 2 - by0) <= (unsigned)(Dy - 5) &&
+                           (unsigned)(Z - 2 - bz0) <= (unsigned)(Dz - 5);
+        if (act && mine && !inBox) scatterGlobal();
+        const int key = (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (X - 2 - bx0);
+        auto addStencil = [&](bool atomic) {
+            float *bq = box + key;
+#pragma unroll
+            for (int cc = 0; cc < 5; cc++)
+#pragma unroll
+                for (int bb = 0; bb < 5; bb++) {
+                    float *rp = bq + cc * PP + bb * RP;
+#pragma unroll
+                    for (int aa = 0; aa < 5; aa++) {
+                        const float w = wi[aa] * wj[bb] * wk[cc];  // the forward gather's weight
+                        if (atomic) {
+                            atomicAdd(rp + aa, w * val);
+                        } else {
+                            rp[aa] = rp[aa] + w * val;
+                            // Keep this tap's write before the next tap's read:
+                            // another lane's stencil may cover this word at a
+                            // later tap, and its read must see this write.
+                            __asm__ volatile("" ::: "memory");
+                        }
+                    }
+                }
+        };
+        bool pend = act && mine && inBox;
+        if (orthoFrame) {
+            // Plain read-modify-write, one tap after the other (LDS float
+            // atomics ran 3.6x slower: 40 vs 11 ms at N=96 with 2,048
+            // directions).  One wave's LDS instructions execute in order and
+            // each tap's read follows the previous tap's write (compiler
+            // barrier), so a lane's read sees every earlier write of every
+            // lane; only lanes of the same instruction can lose an update,
+            // and those target the same word at every tap exactly when their
+            // stencil centres (X, Y, Z) coincide.  For an orthonormal frame two pixels whose rounded
+            // positions coincide differ by less than 1 in every coordinate,
+            // so they are at most sqrt(3) apart: tile neighbours.  A lane waits
+            // while a lower pending neighbour (left, up, up-left, up-right)
+            // has its centre; every round serves at least the lowest pending
+            // lane of each centre.
+            const int ix = ln & 7, jy = ln >> 3;
+            while (__ballot(pend) != 0ull) {
+                const int k = pend ? key : -1 - ln;  // distinct for lanes not pending
+                const int kl = __shfl(k, ln - 1), ku = __shfl(k, ln - 8);
+                const int kul = __shfl(k, ln - 9), kur = __shfl(k, ln - 7);
+                const bool wait = pend && ((ix >= 1 && kl == k) || (jy >= 1 && ku == k) ||
+                                           (ix >= 1 && jy >= 1 && kul == k) || (ix <= 6 && jy >= 1 && kur == k));
+                if (pend && !wait) addStencil(false);
+                pend = wait;
+                wave_lds_sync_fp();
+            }
+        } else if (pend) {
+            addStencil(true);
+        }
+        wave_lds_sync_fp();
+        // Flush the touched voxels plane by plane; a lane takes elements
+        // q = ln, ln + 64, ... of a plane (row ey = q / RP by the reciprocal,
+        // exact for q, RP <= 4096).
+        const float rRP = __builtin_amdgcn_rcpf((float)RP);
+        for (int ez = 0; ez < Dz; ez++) {
+            const float *bp = box + ez * PP;
+            const long long zbase = (long long)(bz0 + ez) * VV + (long long)by0 * V + bx0;
+            for (int q = ln; q < PP; q += 64) {
+                const float s = bp[q];
+                if (s != 0.f) {
+                    const int ey = (int)(((float)q + 0.5f) * rRP);
+                    const int ex = q - ey * RP;
+                    const long long lin = zbase + (long long)ey * V + ex;
+                    if (lin >= 0 && lin < V3) unsafeAtomicAdd(volAcc + lin, s);
+                }
+            }
+        }
+        wave_lds_sync_fp();  // the box is read before it is zeroed again
+        return true;
+    };
+
+    bool done = scatter(0, FPT - 1, true);
+    if (!done) {
+        const bool top = (ln >> 3) < FPT / 2;
+        const bool d0 = scatter(0, FPT / 2 - 1, top);
+        const bool d1 = scatter(FPT / 2, FPT - 1, !top);
+        done = top ? d0 : d1;
+    }
+    if (!done && act) scatterGlobal();
+}
+
+template <bool FAST>
+__global__ __launch_bounds__(kThreads) void adj_lds_kernel(FPArgs p, const float *__restrict__ img,
+                                                           float *__restrict__ volAcc)
+{
+    __shared__ float sker[FPKER];
+    __shared__ float sbox[4][FPBOX];
+    if (!FAST) {
+        for (int t = threadIdx.x; t < p.K; t += kThreads) sker[t] = p.ker[t];
+        __syncthreads();
+    }
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
+    for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
+        adj_lds_tile<FAST>(p, sker, sbox[wv], img, volAcc, work, ln);
+}
+
+// ---------------------------------------------------------------------------
 // Back projection, reference semantics (gpuBackProject.cu:4-113), as an
 // owner-computes gather: a workgroup owns an 8x8x4 voxel brick, culls the
 // directions whose slab |n.(v-c)| <= kerHWidth cannot touch the brick, keeps
@@ -2348,6 +2551,21 @@ hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const f
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
     a.maskR = ref_mask_radius(g.maskRadius);
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
+    if (a.convW == 2 && g.K <= FPKER) {
+        // LDS-box scatter: 8x8 tiles, one per wave, 4 per workgroup.
+        const bool fast = a.kerScale == 125.0f && a.kerCenter == 250.0f && g.K == 501;
+        a.tilesPerRow = (g.M + FPT - 1) / FPT;
+        a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
+        a.maskD2 = mask_d2(a.maskR, g.V);
+        const unsigned long long work = (unsigned long long)a.tilesPerImg * nAxes;
+        if (work > 0xffffffffull / 2) return hipErrorInvalidValue;
+        a.totalWork = (unsigned)work;
+        a.perXcd = 0;
+        const dim3 grid((a.totalWork + 3) / 4), block(kThreads);
+        if (fast) hipLaunchKernelGGL(adj_lds_kernel<true>, grid, block, 0, s, a, casImgs, volAcc);
+        else hipLaunchKernelGGL(adj_lds_kernel<false>, grid, block, 0, s, a, casImgs, volAcc);
+        return hipGetLastError();
+    }
     a.tilesPerRow = (g.M + 15) / 16;
     a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
     const unsigned long long work = (unsigned long long)a.tilesPerImg * nAxes;

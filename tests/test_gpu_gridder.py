@@ -241,8 +241,9 @@ def _check_reconstruction(oracle, cuda, rec, bp, dens, cas_imgs, axes, ker32, n,
     FFT spreads that over the whole volume.  Checked: (1) the device quotient
     against the oracle's on voxels with density > 1e-3 of its peak, at 1e-5;
     (2) that volume rebuilt with the oracle's quotient on the remaining voxels,
-    at 1e-5, which shows the full-volume error comes from those voxels alone;
-    (3) the full reconstruction, at 1e-4, with its error printed."""
+    at 1e-5, which shows how much of the full-volume error comes from those
+    voxels; (3) the full reconstruction, at 1e-5, with its error printed
+    (measured: 1.4e-7 on the n=16 grid case, 2.8e-7 on the C2 subset)."""
     import torch
 
     from multigpugridder_amd import ops
@@ -264,4 +265,4 @@ def _check_reconstruction(oracle, cuda, rec, bp, dens, cas_imgs, axes, ker32, n,
           flush=True)
     assert e_good < TOL
     assert e_mix < TOL
-    assert e_full < 1e-4
+    assert e_full < TOL

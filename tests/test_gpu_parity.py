@@ -256,15 +256,21 @@ def test_back_project_accumulates_and_wraps(cuda, oracle, ker32):
     assert np.count_nonzero(ref) > 0
 
 
-@pytest.mark.parametrize("n,p", [(16, 6), (32, 20)])
-def test_back_project_adjoint(cuda, oracle, ker32, n, p):
+@pytest.mark.parametrize("n,p,scale", [(16, 6, 1.0), (32, 20, 1.0), (48, 40, 1.0), (24, 30, 0.7), (24, 30, 1.3)])
+def test_back_project_adjoint(cuda, oracle, ker32, n, p, scale):
+    """The exact transpose against the oracle's.  n=48 splits most 8x8 tiles
+    into 8x4 halves (their boxes exceed the LDS budget); scaled frames (not
+    orthonormal: many pixels share a stencil centre) take the LDS-atomic
+    branch."""
     import torch
 
     from multigpugridder_amd import ops
     from multigpugridder_amd.synth import random_directions
 
+    print(f"adjoint n={n} p={p} scale={scale}", flush=True)
     m, v = 2 * n, 2 * n + 6
     axes = random_directions(p, 9)
+    axes[1::2] *= np.float32(scale)
     y = np.random.default_rng(3).standard_normal((p, m, m)).astype(np.float32)
     out = torch.zeros((v, v, v), device=cuda)
     ops.back_project_adjoint(_t(y, cuda), _t(axes, cuda), v, ker=_t(ker32, cuda), out=out)
