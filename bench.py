@@ -699,6 +699,16 @@ def main():
         sizes = [int(x) for x in args.serve_sizes.split(",")]
         lams = [float(x) for x in args.serve_lambdas.split(",") if x]
         out["serving"] = serving_bench(sizes, args.serve_requests, lams)
+        if sizes == [n]:
+            # What the kernels allow for this request mix: forward directions at
+            # the step's forward phase rate, back directions at its back phase
+            # rate (device-resident, transfers excluded), against the >= 2x target.
+            sv = out["serving"]
+            fs = sv["forward_share"]
+            bound = 1.0 / (fs / out["fwd_proj_per_s"] + (1.0 - fs) / out["back_proj_per_s"])
+            sv["kernel_bound_dirs_per_s"] = round(bound, 1)
+            sv["ceiling_over_plain"] = round(bound / sv["plain_dirs_per_s"], 3)
+            sv["target_over_plain"] = 2.0
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n, args.cpu_sample, args.cpu_reps)
     print(json.dumps(out), flush=True)

@@ -13,7 +13,12 @@
 //     in launches of up to max_batch_directions, accumulated in order).
 //   * Overlap: num_pipelines batches run at once, each on its own stream, so
 //     the transfers of one batch overlap the kernels of another and small
-//     batches share the device.
+//     batches share the device.  Inside a batch the work runs in sub-chunks of
+//     at most kSub directions, and host transfers go on the pipeline's second
+//     (copy) stream: a forward batch copies each sub-chunk's images out while
+//     the next sub-chunk is projected, a back request uploads the next chunk
+//     of images (double-buffered) while the current one is projected, so a
+//     large request no longer pays its whole PCIe transfer after its kernels.
 //   * Results are those of the plain gridder path for the same request: a
 //     forward request gets forwardProject's images, a back request gets getVol
 //     of the back projection of its images into a zeroed volume.  A request's
@@ -161,9 +166,16 @@ struct Segment {
     int offset;  // position in the batch
 };
 
+// Directions per sub-chunk inside a batch (transfer / compute overlap).
+constexpr int kSub = 1024;
+
 struct Pipeline {
     hipStream_t stream = nullptr;
+    hipStream_t copy = nullptr;  // host transfers of the batch
     hipEvent_t done = nullptr;
+    hipEvent_t handoff[2] = {nullptr, nullptr};  // compute -> copy (sub-chunk ready / buffer free)
+    hipEvent_t copied[2] = {nullptr, nullptr};   // copy -> compute (chunk uploaded)
+    hipEvent_t copyTail = nullptr;              // everything queued on the copy stream
     bool busy = false;
     float *stageAxes = nullptr;  // pinned
     size_t stageAxesLen = 0;
@@ -273,16 +285,18 @@ struct mgg_server {
     }
 
     // ---- device memory of one batch -------------------------------------
+    int sub_size() const { return std::min(maxBatch, kSub); }
     size_t fp_bytes(const Volume &v, int D) const
     {
-        return align_up(sizeof(float) * 9 * (size_t)D) + align_up(sizeof(float) * (size_t)D * v.M * v.M) +
-               align_up(herm2d_bytes(v.M, D)) + align_up(sizeof(float) * (size_t)D * v.N * v.N);
+        const int S = std::min(D, sub_size());
+        return align_up(sizeof(float) * 9 * (size_t)D) + align_up(sizeof(float) * (size_t)S * v.M * v.M) +
+               align_up(herm2d_bytes(v.M, S)) + align_up(sizeof(float) * (size_t)D * v.N * v.N);
     }
     size_t bp_bytes(const Volume &v, int n) const
     {
-        const int C = std::min(n, maxBatch);
+        const int C = std::min(n, sub_size());
         const size_t M = v.M;
-        return align_up(sizeof(float) * 9 * (size_t)n) + align_up(sizeof(float) * (size_t)C * v.N * v.N) +
+        return align_up(sizeof(float) * 9 * (size_t)n) + 2 * align_up(sizeof(float) * (size_t)C * v.N * v.N) +
                align_up(sizeof(float) * (size_t)C * M * M) + align_up(herm2d_bytes(v.M, C)) +
                align_up(sizeof(float) * (size_t)v.V * v.V * v.V) + align_up(sizeof(float) * M * M * M) +
                align_up(sizeof(float2) * M * M * (M / 2 + 1)) + align_up(sizeof(float) * (size_t)v.N * v.N * v.N);
@@ -328,46 +342,62 @@ struct mgg_server {
     }
 
     // ---- issue ------------------------------------------------------------
+    // The copy stream's queued work joins the compute stream (the batch's
+    // completion event is recorded after it).
+    void join_copy(Pipeline &p)
+    {
+        HIP_TRY(hipEventRecord(p.copyTail, p.copy));
+        HIP_TRY(hipStreamWaitEvent(p.stream, p.copyTail, 0));
+    }
+
     void issue_forward(Batch &b)
     {
         const Volume &v = *b.vol;
         Pipeline &p = *b.pipe;
-        const int D = b.n, M = v.M, N = v.N;
+        const int D = b.n, M = v.M, N = v.N, S = std::min(D, sub_size());
         Carver c{(char *)b.mem};
         float *dAxes = c.take<float>(sizeof(float) * 9 * (size_t)D);
-        float *dCas = c.take<float>(sizeof(float) * (size_t)D * M * M);
-        float2 *dHerm = c.take<float2>(herm2d_bytes(M, D));
+        float *dCas = c.take<float>(sizeof(float) * (size_t)S * M * M);
+        float2 *dHerm = c.take<float2>(herm2d_bytes(M, S));
         float *dImgs = c.take<float>(sizeof(float) * (size_t)D * N * N);
         b.dImgs = dImgs;
         upload_axes(p, dAxes, b.segs, D);
         const ProjGeom g = geom(v);
-        check_launch(launch_forward_project(v.cas, DT_F32, g, dAxes, D, dCas, p.stream), "forward_project");
-        // inverse image chain (CASImgsToImgs), groups of kFftBatch images
         const size_t mm = (size_t)M * M, nn = (size_t)N * N, hh = (size_t)M * (M / 2 + 1);
         const float scale = 1.0f / ((float)M * (float)M);
-        for (int a0 = 0; a0 < D; a0 += (int)kFftBatch) {
-            const int gN = std::min((int)kFftBatch, D - a0);
-            if (fused_chain_supported(M, N)) {
-                check_launch(launch_fused_cas_to_images(dCas + a0 * mm, dHerm, dImgs + a0 * nn, M, N, gN, scale,
-                                                        p.stream),
-                             "fused_cas_to_images");
-                continue;
+        int k = 0;
+        for (int s0 = 0; s0 < D; s0 += S, k++) {
+            const int cnt = std::min(S, D - s0);
+            check_launch(launch_forward_project(v.cas, DT_F32, g, dAxes + (size_t)9 * s0, cnt, dCas, p.stream),
+                         "forward_project");
+            // inverse image chain (CASImgsToImgs), groups of kFftBatch images
+            for (int a0 = 0; a0 < cnt; a0 += (int)kFftBatch) {
+                const int gN = std::min((int)kFftBatch, cnt - a0);
+                float *out = dImgs + (size_t)(s0 + a0) * nn;
+                if (fused_chain_supported(M, N)) {
+                    check_launch(launch_fused_cas_to_images(dCas + a0 * mm, dHerm, out, M, N, gN, scale, p.stream),
+                                 "fused_cas_to_images");
+                    continue;
+                }
+                check_launch(launch_cas_to_herm2d(dCas + a0 * mm, dHerm, M, gN, p.stream), "cas_to_herm2d");
+                fft_groups(gN, [&](int a, int cn) { fft_exec(1, M, cn, dHerm + a * hh, dCas + (a0 + a) * mm, p.stream); });
+                check_launch(launch_real2d_to_imgs(dCas + a0 * mm, out, M, N, gN, scale, p.stream), "real2d_to_imgs");
             }
-            check_launch(launch_cas_to_herm2d(dCas + a0 * mm, dHerm, M, gN, p.stream), "cas_to_herm2d");
-            fft_groups(gN, [&](int a, int cnt) { fft_exec(1, M, cnt, dHerm + a * hh, dCas + (a0 + a) * mm, p.stream); });
-            check_launch(launch_real2d_to_imgs(dCas + a0 * mm, dImgs + a0 * nn, M, N, gN, scale, p.stream),
-                         "real2d_to_imgs");
+            // This sub-chunk's images go out on the copy stream while the next
+            // sub-chunk is projected.
+            HIP_TRY(hipEventRecord(p.handoff[k & 1], p.stream));
+            HIP_TRY(hipStreamWaitEvent(p.copy, p.handoff[k & 1], 0));
+            for (const Segment &sg : b.segs) {
+                const int lo = std::max(sg.offset, s0), hi = std::min(sg.offset + sg.count, s0 + cnt);
+                if (lo >= hi || sg.req->imgMem == Mem::Pageable) continue;  // pageable: completion thread
+                float *dst = sg.req->r.images + (size_t)(sg.first + lo - sg.offset) * nn;
+                const size_t bytes = sizeof(float) * (size_t)(hi - lo) * nn;
+                HIP_TRY(hipMemcpyAsync(dst, dImgs + (size_t)lo * nn, bytes,
+                                       sg.req->imgMem == Mem::Device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                       p.copy));
+            }
         }
-        for (const Segment &s : b.segs) {
-            float *dst = s.req->r.images + (size_t)s.first * nn;
-            const float *src = dImgs + (size_t)s.offset * nn;
-            const size_t bytes = sizeof(float) * (size_t)s.count * nn;
-            if (s.req->imgMem == Mem::Device)
-                HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, p.stream));
-            else if (s.req->imgMem == Mem::Pinned)
-                HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, p.stream));
-            // pageable: copied by the completion thread once the batch is done
-        }
+        join_copy(p);
         HIP_TRY(hipEventRecord(p.done, p.stream));
     }
 
@@ -376,11 +406,13 @@ struct mgg_server {
         const Volume &v = *b.vol;
         Pipeline &p = *b.pipe;
         Req &r = *b.segs[0].req;
-        const int n = b.n, M = v.M, N = v.N, C = std::min(n, maxBatch);
+        const int n = b.n, M = v.M, N = v.N, C = std::min(n, sub_size());
         const size_t mm = (size_t)M * M, nn = (size_t)N * N, hh = (size_t)M * (M / 2 + 1);
         Carver c{(char *)b.mem};
         float *dAxes = c.take<float>(sizeof(float) * 9 * (size_t)n);
-        float *dImgs = c.take<float>(sizeof(float) * (size_t)C * nn);
+        float *dImgBuf[2];
+        dImgBuf[0] = c.take<float>(sizeof(float) * (size_t)C * nn);
+        dImgBuf[1] = c.take<float>(sizeof(float) * (size_t)C * nn);
         float *dCas = c.take<float>(sizeof(float) * (size_t)C * mm);
         float2 *dHerm = c.take<float2>(herm2d_bytes(M, C));
         float *dAcc = c.take<float>(sizeof(float) * (size_t)v.V * v.V * v.V);
@@ -391,14 +423,23 @@ struct mgg_server {
         upload_axes(p, dAxes, b.segs, n);
         const ProjGeom g = geom(v);
         check_launch(launch_fill(dAcc, 0.f, (size_t)v.V * v.V * v.V, p.stream), "fill");
-        for (int c0 = 0; c0 < n; c0 += C) {
+        int k = 0;
+        for (int c0 = 0; c0 < n; c0 += C, k++) {
             const int cnt = std::min(C, n - c0);
             const float *src = r.r.images + (size_t)c0 * nn;
-            const float *imgs = dImgs;
-            if (r.imgMem == Mem::Device)
-                imgs = src;
-            else  // pinned: asynchronous; pageable: the runtime stages it
-                HIP_TRY(hipMemcpyAsync(dImgs, src, sizeof(float) * (size_t)cnt * nn, hipMemcpyHostToDevice, p.stream));
+            const float *imgs = src;
+            if (r.imgMem != Mem::Device) {
+                // Upload into the buffer the chunk before last used, once the
+                // compute stream is done with it; double-buffered so the next
+                // chunk's upload overlaps this chunk's kernels.  (Pageable
+                // sources are staged by the runtime.)
+                float *buf = dImgBuf[k & 1];
+                if (k >= 2) HIP_TRY(hipStreamWaitEvent(p.copy, p.handoff[k & 1], 0));
+                HIP_TRY(hipMemcpyAsync(buf, src, sizeof(float) * (size_t)cnt * nn, hipMemcpyHostToDevice, p.copy));
+                HIP_TRY(hipEventRecord(p.copied[k & 1], p.copy));
+                HIP_TRY(hipStreamWaitEvent(p.stream, p.copied[k & 1], 0));
+                imgs = buf;
+            }
             // forward image chain (ImgsToCASImgs), groups of kFftBatch images
             for (int a0 = 0; a0 < cnt; a0 += (int)kFftBatch) {
                 const int gN = std::min((int)kFftBatch, cnt - a0);
@@ -409,9 +450,10 @@ struct mgg_server {
                     continue;
                 }
                 check_launch(launch_imgs_to_real2d(imgs + a0 * nn, dCas + a0 * mm, M, N, gN, p.stream), "imgs_to_real2d");
-                fft_groups(gN, [&](int a, int k) { fft_exec(0, M, k, dCas + (a0 + a) * mm, dHerm + a * hh, p.stream); });
+                fft_groups(gN, [&](int a, int kk) { fft_exec(0, M, kk, dCas + (a0 + a) * mm, dHerm + a * hh, p.stream); });
                 check_launch(launch_herm2d_to_cas(dHerm, dCas + a0 * mm, M, gN, p.stream), "herm2d_to_cas");
             }
+            if (r.imgMem != Mem::Device) HIP_TRY(hipEventRecord(p.handoff[k & 1], p.stream));  // buffer free
             check_launch(back_project_launch(dAcc, nullptr, g, dAxes + (size_t)9 * c0, cnt, dCas, p.stream),
                          "back_project");
         }
@@ -421,6 +463,7 @@ struct mgg_server {
             HIP_TRY(hipMemcpyAsync(r.r.volume, dVol, vbytes, hipMemcpyDeviceToDevice, p.stream));
         else if (r.volMem == Mem::Pinned)
             HIP_TRY(hipMemcpyAsync(r.r.volume, dVol, vbytes, hipMemcpyDeviceToHost, p.stream));
+        join_copy(p);
         HIP_TRY(hipEventRecord(p.done, p.stream));
     }
 
@@ -564,8 +607,10 @@ struct mgg_server {
             }
             if (code != MGG_OK) {
                 // Nothing more is queued for the batch; the event covers what
-                // was queued before the failure, so its memory is released
-                // only after that work.
+                // was queued before the failure on both streams, so its memory
+                // is released only after that work.
+                if (hipEventRecord(raw->pipe->copyTail, raw->pipe->copy) == hipSuccess)
+                    (void)hipStreamWaitEvent(raw->pipe->stream, raw->pipe->copyTail, 0);
                 (void)hipEventRecord(raw->pipe->done, raw->pipe->stream);
             }
             lk.lock();
@@ -671,7 +716,13 @@ struct mgg_server {
         for (int i = 0; i < nPipes; i++) {
             auto p = std::make_unique<Pipeline>();
             HIP_TRY(hipStreamCreateWithFlags(&p->stream, hipStreamNonBlocking));
+            HIP_TRY(hipStreamCreateWithFlags(&p->copy, hipStreamNonBlocking));  // default priority
             HIP_TRY(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&p->copyTail, hipEventDisableTiming));
+            for (int e = 0; e < 2; e++) {
+                HIP_TRY(hipEventCreateWithFlags(&p->handoff[e], hipEventDisableTiming));
+                HIP_TRY(hipEventCreateWithFlags(&p->copied[e], hipEventDisableTiming));
+            }
             pipes.push_back(std::move(p));
         }
         scheduler = std::thread([this] {
@@ -702,7 +753,16 @@ struct mgg_server {
                 fft_drop_stream(p->stream);
                 (void)hipStreamDestroy(p->stream);
             }
+            if (p->copy) {
+                (void)hipStreamSynchronize(p->copy);
+                (void)hipStreamDestroy(p->copy);
+            }
             if (p->done) (void)hipEventDestroy(p->done);
+            if (p->copyTail) (void)hipEventDestroy(p->copyTail);
+            for (int e = 0; e < 2; e++) {
+                if (p->handoff[e]) (void)hipEventDestroy(p->handoff[e]);
+                if (p->copied[e]) (void)hipEventDestroy(p->copied[e]);
+            }
             if (p->stageAxes) (void)hipHostFree(p->stageAxes);
         }
         if (adminStream) {

@@ -254,3 +254,62 @@ def test_server_wait_timeout_then_result(cuda):
             info = s.wait(t)
             assert info.t_done >= info.t_start >= info.t_submit
         assert np.abs(out).max() > 0
+
+
+def test_server_c5_shape_same_alone_and_in_mix(cuda):
+    """The serving configuration's shape: one resident blob volume for each of
+    N = 64, 96, 128, 192 and 256 (interp 2), 30 requests, 60% forward / 40%
+    back, direction counts log-uniform in [64, 8192] (random quaternions).
+    Every request's result is the same run alone on an idle server and in the
+    concurrent mix (forward batches merge across requests and are projected in
+    sub-chunks with their transfers overlapped; back requests upload
+    double-buffered chunks).  Half the host buffers are pinned, half pageable."""
+    from multigpugridder_amd import Server, blob_volume, pinned_empty, random_directions
+
+    sizes = [64, 96, 128, 192, 256]
+    rng = np.random.default_rng(6)
+    reqs = []
+    for i in range(30):
+        vi = int(rng.integers(len(sizes)))
+        kind = "fp" if rng.random() < 0.6 else "bp"
+        k = int(round(float(np.exp(rng.uniform(np.log(64), np.log(8192))))))
+        reqs.append((kind, vi, k, 2000 + i))
+    print(f"C5-shaped mix: {len(reqs)} requests, {sum(r[2] for r in reqs)} directions", flush=True)
+    with Server(budget_bytes=48 << 30, num_pipelines=4) as s:
+        vids = [s.add_volume(blob_volume(nn, 10 + i)) for i, nn in enumerate(sizes)]
+        inputs = []
+        for j, (kind, vi, k, seed) in enumerate(reqs):
+            nn = sizes[vi]
+            axes = random_directions(k, seed)
+            imgs = None
+            if kind == "bp":
+                imgs = pinned_empty((k, nn, nn)) if j % 2 else np.empty((k, nn, nn), np.float32)
+                imgs[:] = np.random.default_rng(seed).standard_normal((k, nn, nn), dtype=np.float32)
+            inputs.append((kind, vi, axes, imgs))
+
+        def submit(j):
+            kind, vi, axes, imgs = inputs[j]
+            nn = sizes[vi]
+            if kind == "fp":
+                out = pinned_empty((axes.shape[0], nn, nn)) if j % 2 else np.empty((axes.shape[0], nn, nn), np.float32)
+                return s.submit_forward(vids[vi], axes, out), out
+            out = pinned_empty((nn, nn, nn)) if j % 2 else np.empty((nn, nn, nn), np.float32)
+            return s.submit_back(vids[vi], axes, imgs, out), out
+
+        print("C5 alone pass", flush=True)
+        alone = []
+        for j in range(len(reqs)):
+            t, out = submit(j)
+            s.wait(t)
+            alone.append(out)
+        print("C5 mixed pass", flush=True)
+        mixed = [submit(j) for j in range(len(reqs))]
+        worst = 0.0
+        for j, (t, out) in enumerate(mixed):
+            s.wait(t)
+            worst = max(worst, rel_l2(out, alone[j]))
+        st = s.stats()
+    print(f"C5 alone vs mix: worst relative difference {worst:.2e}; {st['batches']} batches, "
+          f"{st['merged_batches']} merged", flush=True)
+    assert worst <= 1e-6
+    assert st["failed"] == 0 and st["completed"] == 2 * len(reqs)
