@@ -119,6 +119,8 @@ _SIGS = {
     "mgg_kb_table": [_vp, _ci, _ci],
     "mgg_precomp": [_vp, _ci, _ci],
     "mgg_uniform_axes": [_vp, _ci, _ci],
+    # MEX dispatch layer (include/mgg_mex.h)
+    "mgg_mex_call": [ctypes.c_char_p, _ci, _ci, _vp, ctypes.POINTER(ctypes.c_uint64), ctypes.c_char_p, _sz],
     # request server
     "mgg_server_create": [_vp, ctypes.POINTER(_vp)],
     "mgg_server_destroy": [_vp],

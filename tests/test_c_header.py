@@ -1,4 +1,4 @@
-"""The public header is plain C: a C99 program that uses every declared type
+"""The public headers are plain C: a C99 program that uses every declared type
 and takes the address of every declared function compiles with -Wall -Werror
 and links against libMultiGPUGridder.so (nothing is run, no GPU needed)."""
 import os
@@ -16,7 +16,8 @@ LIB_DIR = os.path.join(ROOT, "multigpugridder_amd")
 @pytest.mark.skipif(shutil.which("gcc") is None, reason="no C compiler")
 def test_header_compiles_and_links_as_c99(tmp_path):
     names = header_functions()
-    src = ['#include "mgg.h"', "#include <stdio.h>", "int main(void) {",
+    src = ['#include "mgg.h"', '#include "mgg_mex.h"', "#include <stdio.h>", "int main(void) {",
+           "    mgg_mex_arg marg = {0}; (void)marg;",
            "    mgg_server_config cfg = {0}; mgg_request req = {0}; mgg_request_info info = {0};",
            "    mgg_server_stats st = {0}; mgg_gridder *g = NULL; mgg_server *s = NULL;",
            "    (void)cfg; (void)req; (void)info; (void)st; (void)g; (void)s;",

@@ -13,11 +13,12 @@ from conftest import ROOT, gpu_available
 import multigpugridder_amd as mgg
 from multigpugridder_amd import _lib
 
-HEADER = os.path.join(ROOT, "include", "mgg.h")
+HEADERS = [os.path.join(ROOT, "include", h) for h in sorted(os.listdir(os.path.join(ROOT, "include")))
+           if h.endswith(".h")]
 
 
 def header_functions():
-    text = open(HEADER).read()
+    text = "\n".join(open(h).read() for h in HEADERS)
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
     names = re.findall(r"\b([A-Za-z_][A-Za-z0-9_]*)\s*\(", text)
     skip = {"defined", "if", "sizeof"}
