@@ -37,6 +37,32 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
 hipError_t launch_back_project(float *volAcc, float *densAcc, const ProjGeom &g,
                                const float *axes, int nAxes, const float *casImgs,
                                hipStream_t s, const int *brickOrder, int orderLen);
+// Forward projection over several volumes in one launch (FP32 storage, the
+// 501-entry table): each segment is a run of directions on one volume,
+// written to its own CAS images; work items are the segments' 8x8 pixel
+// tiles, numbered segment after segment from workStart.
+constexpr int kFpMaxSegments = 16;
+struct FPSegment {
+    const void *vol;
+    const float *axes;
+    float *out;
+    const float *ker;
+    int V, M, nAxes, maskD2, tilesPerRow, tilesPerImg;
+    unsigned workStart;
+};
+struct FPMultiArgs {
+    FPSegment seg[kFpMaxSegments];
+    int nSegs;
+    unsigned totalWork;
+};
+// True when a volume of this geometry can be a segment (kernel half width 2,
+// the 501-entry table).
+bool fp_segment_supported(const ProjGeom &g);
+// Fills sg for nAxes directions of the volume vol (FP32) and returns its work
+// items (tiles); the caller sets the next segment's workStart past them.
+unsigned fp_segment(FPSegment &sg, const void *vol, const ProjGeom &g, const float *axes, int nAxes,
+                    float *casImgs, unsigned workStart);
+hipError_t launch_forward_project_multi(const FPMultiArgs &m, hipStream_t s);
 // The fast back projection walks bricks in a centre-out order; these give the
 // table length (0 when the generic kernel is used) and fill it (host memory).
 int bp_brick_count(int V, bool fast);

@@ -143,6 +143,8 @@ struct FPArgs {
     int maskD2;  // fp_lds_kernel: largest integer d2 with sqrtf((float)d2) <= maskR (-1: none)
 };
 
+constexpr int KFP_FAST = 501;  // the table the mixed-volume launch requires (kerScale 125, kerCenter 250)
+
 template <int DT, bool W2>
 __global__ __launch_bounds__(kThreads) void fp_kernel(FPArgs p)
 {
@@ -682,6 +684,38 @@ __global__ __launch_bounds__(kThreads) void adj_kernel(FPArgs p, const float *__
                 if (lin < V3 && lin >= 0 && w != 0.f) unsafeAtomicAdd(volAcc + lin, w * val);
             }
         }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Forward projection over several volumes in one launch (the request
+// server's batches mix volumes of different N).  Each segment is a run of
+// directions on one volume with its own geometry and output; work items are
+// the segments' 8x8 tiles, numbered segment after segment, and every wave
+// finds its segment by a binary search over the segments' first work items
+// (wave-uniform, so the search and the descriptor reads are scalar).  The
+// tile code is fp_lds_tile's, so each image is bit-identical to a launch of
+// its own volume alone.
+// ---------------------------------------------------------------------------
+template <int DT>
+__global__ __launch_bounds__(kThreads) void fp_lds_multi_kernel(FPMultiArgs m)
+{
+    __shared__ __attribute__((aligned(16))) float sbox[4][FPBOX];
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
+    for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < m.totalWork; work += gridDim.x * 4u) {
+        int lo = 0, hi = m.nSegs - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (m.seg[mid].workStart <= work) lo = mid; else hi = mid - 1;
+        }
+        const FPSegment &sg = m.seg[lo];
+        FPArgs a;
+        a.vol = sg.vol; a.axes = sg.axes; a.out = sg.out; a.ker = sg.ker;
+        a.V = sg.V; a.M = sg.M; a.K = KFP_FAST; a.nAxes = sg.nAxes;
+        a.maskR = 0.f; a.kerScale = 125.0f; a.kerCenter = 250.0f; a.fW = 2.0f;
+        a.convW = 2; a.tilesPerRow = sg.tilesPerRow; a.tilesPerImg = sg.tilesPerImg;
+        a.perXcd = 0; a.totalWork = 0; a.maskD2 = sg.maskD2;
+        fp_lds_tile<DT, true>(a, nullptr, sbox[wv], work - sg.workStart, ln);
     }
 }
 
@@ -2502,6 +2536,34 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
         break;
     default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+bool fp_segment_supported(const ProjGeom &g)
+{
+    float kerScale, kerCenter, fW;
+    int convW;
+    proj_consts(g, kerScale, kerCenter, fW, convW);
+    return convW == 2 && kerScale == 125.0f && kerCenter == 250.0f && g.K == KFP_FAST && g.K <= FPKER;
+}
+
+unsigned fp_segment(FPSegment &sg, const void *vol, const ProjGeom &g, const float *axes, int nAxes, float *casImgs,
+                    unsigned workStart)
+{
+    sg.vol = vol; sg.axes = axes; sg.out = casImgs; sg.ker = g.ker;
+    sg.V = g.V; sg.M = g.M; sg.nAxes = nAxes;
+    sg.maskD2 = mask_d2(ref_mask_radius(g.maskRadius), g.V);
+    sg.tilesPerRow = (g.M + FPT - 1) / FPT;
+    sg.tilesPerImg = sg.tilesPerRow * sg.tilesPerRow;
+    sg.workStart = workStart;
+    return (unsigned)sg.tilesPerImg * (unsigned)nAxes;
+}
+
+hipError_t launch_forward_project_multi(const FPMultiArgs &m, hipStream_t s)
+{
+    if (m.nSegs <= 0 || m.totalWork == 0) return hipSuccess;
+    if (m.nSegs > kFpMaxSegments) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(fp_lds_multi_kernel<DT_F32>, dim3((m.totalWork + 3) / 4), dim3(kThreads), 0, s, m);
     return hipGetLastError();
 }
 

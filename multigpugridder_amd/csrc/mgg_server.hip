@@ -6,9 +6,12 @@
 //   * Admission: every request's device memory (per batch) comes out of one
 //     arena of budget_bytes.  A batch starts when a pipeline is free and the
 //     arena can hold it; otherwise it waits for running batches to finish.
-//   * Batching: forward requests on the same volume are merged into one
-//     launch of up to max_batch_directions directions, across requests and
-//     across request boundaries (a large request spans several batches).  A
+//   * Batching: forward requests are merged into one batch of up to
+//     max_batch_directions directions, across requests, across request
+//     boundaries (a large request spans several batches) and across resident
+//     volumes of different N: each sub-chunk of a batch is one forward
+//     projection launch over work items of every volume in it
+//     (launch_forward_project_multi), followed by each volume's image chain.  A
 //     back request owns its accumulator and runs as one batch (its directions
 //     in launches of up to max_batch_directions, accumulated in order).
 //   * Overlap: num_pipelines batches run at once, each on its own stream, so
@@ -186,9 +189,10 @@ struct Batch {
     Volume *vol = nullptr;
     int n = 0;
     std::vector<Segment> segs;
+    std::vector<size_t> imgBase;  // forward: first float of each segment's images in dImgs
     void *mem = nullptr;  // one arena block, subdivided
     Pipeline *pipe = nullptr;
-    float *dImgs = nullptr;  // forward: D x N^2 images; back: N^3 volume
+    float *dImgs = nullptr;  // forward: every segment's images (N^2 each); back: N^3 volume
     int failed = MGG_OK;
     std::string err;
     bool ready = false;  // everything of the batch is queued on its stream
@@ -292,6 +296,19 @@ struct mgg_server {
         return align_up(sizeof(float) * 9 * (size_t)D) + align_up(sizeof(float) * (size_t)S * v.M * v.M) +
                align_up(herm2d_bytes(v.M, S)) + align_up(sizeof(float) * (size_t)D * v.N * v.N);
     }
+    // A forward batch whose segments may be on different volumes.
+    size_t fp_bytes(const std::vector<Segment> &segs, int D) const
+    {
+        const int S = std::min(D, sub_size());
+        int Mmax = 0;
+        size_t imgs = 0;
+        for (const Segment &sg : segs) {
+            Mmax = std::max(Mmax, sg.req->vol->M);
+            imgs += (size_t)sg.count * sg.req->vol->N * sg.req->vol->N;
+        }
+        return align_up(sizeof(float) * 9 * (size_t)D) + align_up(sizeof(float) * (size_t)S * Mmax * Mmax) +
+               align_up(herm2d_bytes(Mmax, S)) + align_up(sizeof(float) * imgs);
+    }
     size_t bp_bytes(const Volume &v, int n) const
     {
         const int C = std::min(n, sub_size());
@@ -352,47 +369,87 @@ struct mgg_server {
 
     void issue_forward(Batch &b)
     {
-        const Volume &v = *b.vol;
         Pipeline &p = *b.pipe;
-        const int D = b.n, M = v.M, N = v.N, S = std::min(D, sub_size());
+        const int D = b.n, S = std::min(D, sub_size());
+        int Mmax = 0;
+        for (const Segment &sg : b.segs) Mmax = std::max(Mmax, sg.req->vol->M);
         Carver c{(char *)b.mem};
         float *dAxes = c.take<float>(sizeof(float) * 9 * (size_t)D);
-        float *dCas = c.take<float>(sizeof(float) * (size_t)S * M * M);
-        float2 *dHerm = c.take<float2>(herm2d_bytes(M, S));
-        float *dImgs = c.take<float>(sizeof(float) * (size_t)D * N * N);
+        float *dCas = c.take<float>(sizeof(float) * (size_t)S * Mmax * Mmax);
+        float2 *dHerm = c.take<float2>(herm2d_bytes(Mmax, S));
+        b.imgBase.clear();
+        size_t imgFloats = 0;
+        for (const Segment &sg : b.segs) {
+            b.imgBase.push_back(imgFloats);
+            imgFloats += (size_t)sg.count * sg.req->vol->N * sg.req->vol->N;
+        }
+        float *dImgs = c.take<float>(sizeof(float) * imgFloats);
         b.dImgs = dImgs;
         upload_axes(p, dAxes, b.segs, D);
-        const ProjGeom g = geom(v);
-        const size_t mm = (size_t)M * M, nn = (size_t)N * N, hh = (size_t)M * (M / 2 + 1);
-        const float scale = 1.0f / ((float)M * (float)M);
         int k = 0;
         for (int s0 = 0; s0 < D; s0 += S, k++) {
             const int cnt = std::min(S, D - s0);
-            check_launch(launch_forward_project(v.cas, DT_F32, g, dAxes + (size_t)9 * s0, cnt, dCas, p.stream),
-                         "forward_project");
-            // inverse image chain (CASImgsToImgs), groups of kFftBatch images
-            for (int a0 = 0; a0 < cnt; a0 += (int)kFftBatch) {
-                const int gN = std::min((int)kFftBatch, cnt - a0);
-                float *out = dImgs + (size_t)(s0 + a0) * nn;
-                if (fused_chain_supported(M, N)) {
-                    check_launch(launch_fused_cas_to_images(dCas + a0 * mm, dHerm, out, M, N, gN, scale, p.stream),
-                                 "fused_cas_to_images");
-                    continue;
+            // The pieces of this sub-chunk: each segment's overlap with it,
+            // their CAS images packed one after the other in dCas.
+            struct Piece { int seg, lo, hi; size_t cas; };
+            std::vector<Piece> pieces;
+            size_t casOff = 0;
+            for (int i = 0; i < (int)b.segs.size(); i++) {
+                const Segment &sg = b.segs[i];
+                const int lo = std::max(sg.offset, s0), hi = std::min(sg.offset + sg.count, s0 + cnt);
+                if (lo >= hi) continue;
+                pieces.push_back({i, lo, hi, casOff});
+                const size_t M = (size_t)sg.req->vol->M;
+                casOff += (size_t)(hi - lo) * M * M;
+            }
+            // One forward projection launch for up to kFpMaxSegments pieces,
+            // whatever their volumes.
+            for (size_t p0 = 0; p0 < pieces.size(); p0 += kFpMaxSegments) {
+                FPMultiArgs ma;
+                ma.nSegs = 0;
+                unsigned work = 0;
+                for (size_t q = p0; q < pieces.size() && q < p0 + kFpMaxSegments; q++) {
+                    const Piece &pc = pieces[q];
+                    const Volume &v = *b.segs[pc.seg].req->vol;
+                    work += fp_segment(ma.seg[ma.nSegs++], v.cas, geom(v), dAxes + (size_t)9 * pc.lo, pc.hi - pc.lo,
+                                       dCas + pc.cas, work);
                 }
-                check_launch(launch_cas_to_herm2d(dCas + a0 * mm, dHerm, M, gN, p.stream), "cas_to_herm2d");
-                fft_groups(gN, [&](int a, int cn) { fft_exec(1, M, cn, dHerm + a * hh, dCas + (a0 + a) * mm, p.stream); });
-                check_launch(launch_real2d_to_imgs(dCas + a0 * mm, out, M, N, gN, scale, p.stream), "real2d_to_imgs");
+                ma.totalWork = work;
+                check_launch(launch_forward_project_multi(ma, p.stream), "forward_project_multi");
+            }
+            // Each piece's inverse image chain (CASImgsToImgs), groups of kFftBatch images.
+            for (const Piece &pc : pieces) {
+                const Segment &sg = b.segs[pc.seg];
+                const int M = sg.req->vol->M, N = sg.req->vol->N, n = pc.hi - pc.lo;
+                const size_t mm = (size_t)M * M, nn = (size_t)N * N, hh = (size_t)M * (M / 2 + 1);
+                const float scale = 1.0f / ((float)M * (float)M);
+                float *cas = dCas + pc.cas;
+                float *out = dImgs + b.imgBase[pc.seg] + (size_t)(pc.lo - sg.offset) * nn;
+                for (int a0 = 0; a0 < n; a0 += (int)kFftBatch) {
+                    const int gN = std::min((int)kFftBatch, n - a0);
+                    if (fused_chain_supported(M, N)) {
+                        check_launch(launch_fused_cas_to_images(cas + a0 * mm, dHerm, out + a0 * nn, M, N, gN, scale,
+                                                                p.stream),
+                                     "fused_cas_to_images");
+                        continue;
+                    }
+                    check_launch(launch_cas_to_herm2d(cas + a0 * mm, dHerm, M, gN, p.stream), "cas_to_herm2d");
+                    fft_groups(gN, [&](int a, int cn) { fft_exec(1, M, cn, dHerm + a * hh, cas + (a0 + a) * mm, p.stream); });
+                    check_launch(launch_real2d_to_imgs(cas + a0 * mm, out + a0 * nn, M, N, gN, scale, p.stream),
+                                 "real2d_to_imgs");
+                }
             }
             // This sub-chunk's images go out on the copy stream while the next
             // sub-chunk is projected.
             HIP_TRY(hipEventRecord(p.handoff[k & 1], p.stream));
             HIP_TRY(hipStreamWaitEvent(p.copy, p.handoff[k & 1], 0));
-            for (const Segment &sg : b.segs) {
-                const int lo = std::max(sg.offset, s0), hi = std::min(sg.offset + sg.count, s0 + cnt);
-                if (lo >= hi || sg.req->imgMem == Mem::Pageable) continue;  // pageable: completion thread
-                float *dst = sg.req->r.images + (size_t)(sg.first + lo - sg.offset) * nn;
-                const size_t bytes = sizeof(float) * (size_t)(hi - lo) * nn;
-                HIP_TRY(hipMemcpyAsync(dst, dImgs + (size_t)lo * nn, bytes,
+            for (const Piece &pc : pieces) {
+                const Segment &sg = b.segs[pc.seg];
+                if (sg.req->imgMem == Mem::Pageable) continue;  // the completion thread copies it
+                const size_t nn = (size_t)sg.req->vol->N * sg.req->vol->N;
+                float *dst = sg.req->r.images + (size_t)(sg.first + pc.lo - sg.offset) * nn;
+                const float *src = dImgs + b.imgBase[pc.seg] + (size_t)(pc.lo - sg.offset) * nn;
+                HIP_TRY(hipMemcpyAsync(dst, src, sizeof(float) * (size_t)(pc.hi - pc.lo) * nn,
                                        sg.req->imgMem == Mem::Device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
                                        p.copy));
             }
@@ -470,17 +527,19 @@ struct mgg_server {
     // Host copies into pageable result arrays, after the batch finished.
     void finish_host_copies(Batch &b)
     {
-        const Volume &v = *b.vol;
-        const size_t nn = (size_t)v.N * v.N;
         if (b.kind == MGG_REQ_FORWARD) {
-            for (const Segment &s : b.segs)
+            for (size_t i = 0; i < b.segs.size(); i++) {
+                const Segment &s = b.segs[i];
+                const size_t sn = (size_t)s.req->vol->N * s.req->vol->N;
                 if (s.req->imgMem == Mem::Pageable)
-                    HIP_TRY(hipMemcpy(s.req->r.images + (size_t)s.first * nn, b.dImgs + (size_t)s.offset * nn,
-                                      sizeof(float) * (size_t)s.count * nn, hipMemcpyDeviceToHost));
+                    HIP_TRY(hipMemcpy(s.req->r.images + (size_t)s.first * sn, b.dImgs + b.imgBase[i],
+                                      sizeof(float) * (size_t)s.count * sn, hipMemcpyDeviceToHost));
+            }
         } else {
+            const Volume &v = *b.vol;  // a back request's own volume
             Req &r = *b.segs[0].req;
             if (r.volMem == Mem::Pageable)
-                HIP_TRY(hipMemcpy(r.r.volume, b.dImgs, sizeof(float) * nn * v.N, hipMemcpyDeviceToHost));
+                HIP_TRY(hipMemcpy(r.r.volume, b.dImgs, sizeof(float) * v.N * v.N * v.N, hipMemcpyDeviceToHost));
         }
     }
 
@@ -528,17 +587,18 @@ struct mgg_server {
             b->segs.push_back({h, 0, h->r.n_axes, 0});
             b->mem = arena.alloc(bp_bytes(*h->vol, h->r.n_axes));
         } else {
+            // Forward requests of every volume, in queue order.
             int D = 0;
             for (Req *r : queue) {
                 if (D >= maxBatch) break;
-                if (r->r.kind != MGG_REQ_FORWARD || r->vol != h->vol) continue;
+                if (r->r.kind != MGG_REQ_FORWARD) continue;
                 const int take = std::min(r->r.n_axes - r->issued, maxBatch - D);
                 if (take <= 0) continue;
                 b->segs.push_back({r, r->issued, take, D});
                 D += take;
             }
             b->n = D;
-            b->mem = arena.alloc(fp_bytes(*h->vol, D));
+            b->mem = arena.alloc(fp_bytes(b->segs, D));
         }
         if (!b->mem) return nullptr;
         const double t = now();
