@@ -505,6 +505,7 @@ def main():
 
     from multigpugridder_amd import ops
     from multigpugridder_amd.hostmath import uniform_axes
+    from multigpugridder_amd.pipeline import ProjectionPipeline
     from multigpugridder_amd.synth import blob_volume
 
     if args.serve_only:
@@ -529,67 +530,22 @@ def main():
     ker = ops.kb_table(dev)
     cas_vol = ops.volume_to_cas_volume(vol)
     nst = max(1, args.streams)
-    streams = [torch.cuda.Stream(device=dev) for _ in range(nst)]  # default priority
-    cas_bufs = [torch.empty((chunk, m, m), device=dev) for _ in range(nst)]
-    imgs = torch.empty((p, n, n), device=dev)
-    bp_vol = torch.zeros((v, v, v), device=dev)
+    pipe = ProjectionPipeline(cas_vol, axes, n, chunk, nst, ker)
     torch.cuda.synchronize()
 
     ev = {"fp": [], "bp": [], "fp_chain": [], "bp_chain": []}
 
-    def timed(name, fn, record, st):
+    def timed(name, fn, st):
         # HIP events on the stream the kernels run on
-        with torch.cuda.stream(st):
-            if record:
-                a = torch.cuda.Event(enable_timing=True)
-                b = torch.cuda.Event(enable_timing=True)
-                a.record(st)
-                fn()
-                b.record(st)
-                ev[name].append((a, b))
-            else:
-                fn()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(st)
+        fn()
+        b.record(st)
+        ev[name].append((a, b))
 
     def step(record):
-        # Chunk k runs on stream k % nst with its own CAS-image buffer, so the
-        # HBM-bound image chain of one chunk overlaps the projector of the
-        # next.  Projector launches are chained with events (back projection
-        # read-modify-writes one volume; keeping the forward launches apart
-        # too keeps each launch's measured time its own).
-        main = torch.cuda.current_stream()
-        go = torch.cuda.Event()
-        go.record(main)
-        for st in streams:
-            st.wait_event(go)
-        chunks = list(range(0, p, chunk))
-        prev = None
-        for k, s0 in enumerate(chunks):
-            c = min(chunk, p - s0)
-            ax = axes[s0:s0 + c]
-            st, buf = streams[k % nst], cas_bufs[k % nst]
-            if prev is not None:
-                st.wait_event(prev)  # projector launches do not overlap each other
-            timed("fp", lambda: ops.forward_project(cas_vol, ax, m, ker=ker, out=buf[:c]), record, st)
-            prev = torch.cuda.Event()
-            prev.record(st)
-            timed("fp_chain", lambda: ops.cas_images_to_images(buf[:c], n, out=imgs[s0:s0 + c], clobber=True),
-                  record, st)
-        with torch.cuda.stream(streams[0]):
-            bp_vol.zero_()
-        for k, s0 in enumerate(chunks):
-            c = min(chunk, p - s0)
-            ax = axes[s0:s0 + c]
-            st, buf = streams[k % nst], cas_bufs[k % nst]
-            timed("bp_chain", lambda: ops.images_to_cas_images(imgs[s0:s0 + c], m, out=buf[:c]), record, st)
-            if prev is not None:
-                st.wait_event(prev)
-            timed("bp", lambda: ops.back_project(buf[:c], ax, v, ker=ker, out=bp_vol), record, st)
-            prev = torch.cuda.Event()
-            prev.record(st)
-        for st in streams:
-            done = torch.cuda.Event()
-            done.record(st)
-            main.wait_event(done)
+        pipe.step(timed if record else None)
 
     if args.calib_copy_mb:
         src = torch.ones(args.calib_copy_mb * (1 << 20) // 4, device=dev)
@@ -689,8 +645,8 @@ def main():
             "source": "VALU instructions per direction: rocprofv3 SQ_INSTS_VALU (profiles/issue_kernels_n128.md); "
                       "peak: peak_probe on this GPU (independent FP32 adds)"}
     if not args.no_extras:
-        out["reconstruct"] = reconstruct_bench(dev, imgs, axes, ker, n, chunk)
-    del cas_bufs, imgs, bp_vol
+        out["reconstruct"] = reconstruct_bench(dev, pipe.imgs, axes, ker, n, chunk)
+    del pipe
     torch.cuda.empty_cache()
     if not args.no_extras:
         out["dropin"] = dropin_bench(n, axes_np)

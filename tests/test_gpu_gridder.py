@@ -219,6 +219,14 @@ def test_config_c2_n128_grid_subset(cuda, oracle, ker32):
     imgs = g.forwardProject(axes)
     cas, ref = _oracle_forward(oracle, ker32, vol, axes, n)
     assert rel_l2(imgs, ref) < TOL
+    # the gather's own output at this geometry, with the max-abs guard: a
+    # flipped rounded position or table index in any pixel fails
+    import torch
+
+    from multigpugridder_amd import ops
+    ci = ops.forward_project(torch.from_numpy(cas).to(cuda), torch.from_numpy(axes).to(cuda), m,
+                             ker=torch.from_numpy(ker32).to(cuda)).cpu().numpy()
+    assert_parity(ci, oracle.forward_project(cas, axes, ker32, m))
     g.resetVolume()
     g.backProject(imgs, axes)
     cas_vol = g.casVolumeToHost()
