@@ -695,7 +695,8 @@ __global__ __launch_bounds__(kThreads) void adj_kernel(FPArgs p, const float *__
 // finds its segment by a binary search over the segments' first work items
 // (wave-uniform, so the search and the descriptor reads are scalar).  The
 // tile code is fp_lds_tile's, so each image is bit-identical to a launch of
-// its own volume alone.
+// its own volume alone.  The launch is persistent (no more workgroups than are
+// resident at once), so a batch of small requests is one long-running launch.
 // ---------------------------------------------------------------------------
 template <int DT>
 __global__ __launch_bounds__(kThreads) void fp_lds_multi_kernel(FPMultiArgs m)
@@ -2563,7 +2564,16 @@ hipError_t launch_forward_project_multi(const FPMultiArgs &m, hipStream_t s)
 {
     if (m.nSegs <= 0 || m.totalWork == 0) return hipSuccess;
     if (m.nSegs > kFpMaxSegments) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(fp_lds_multi_kernel<DT_F32>, dim3((m.totalWork + 3) / 4), dim3(kThreads), 0, s, m);
+    // Persistent: at most as many workgroups as the GPU holds at once (four per
+    // CU: 40 KB of LDS and four waves per SIMD each); every workgroup walks the
+    // work list until it is exhausted.
+    int dev = 0, cus = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return e;
+    const unsigned resident = 4u * (unsigned)std::max(cus, 1);
+    const unsigned grid = std::min((m.totalWork + 3) / 4, resident);
+    hipLaunchKernelGGL(fp_lds_multi_kernel<DT_F32>, dim3(grid), dim3(kThreads), 0, s, m);
     return hipGetLastError();
 }
 
