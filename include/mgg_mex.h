@@ -14,8 +14,10 @@
  * Handle convention (Mex/mexFunctionWrapper.h:29-82): mexCreateGridder
  * returns a real uint64 scalar holding the address of a handle record with
  * the signature 0xFF00F0A5 and a type name; every other entry point checks
- * that the argument is a real uint64 scalar, that the signature and the type
- * name match, and fails with "Handle not valid." otherwise.
+ * that the argument is a real uint64 scalar naming a live handle (one created
+ * and not yet deleted; the reference reads whatever the integer points at),
+ * that the signature and the type name match, and fails with "Handle not
+ * valid." otherwise.
  */
 #ifndef MGG_MEX_H
 #define MGG_MEX_H

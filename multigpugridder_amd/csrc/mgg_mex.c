@@ -24,6 +24,45 @@ typedef struct mgg_mex_handle {
 
 static const char kTypeName[] = "mgg_gridder";
 
+/* Handles that mexCreateGridder returned and mexDeleteGridder has not freed.
+ * A handle is dereferenced only after it is found here, so a stale or
+ * arbitrary uint64 is refused instead of read (the reference reads whatever
+ * the integer points at, Mex/mexFunctionWrapper.h:70-73).  MEX files run on
+ * Matlab's one interpreter thread. */
+typedef struct live_node {
+    mgg_mex_handle *h;
+    struct live_node *next;
+} live_node;
+static live_node *g_live;
+
+static int live_add(mgg_mex_handle *h)
+{
+    live_node *n = (live_node *)malloc(sizeof *n);
+    if (!n) return 0;
+    n->h = h;
+    n->next = g_live;
+    g_live = n;
+    return 1;
+}
+
+static int live_has(const mgg_mex_handle *h)
+{
+    for (const live_node *n = g_live; n; n = n->next)
+        if (n->h == h) return 1;
+    return 0;
+}
+
+static void live_remove(const mgg_mex_handle *h)
+{
+    for (live_node **pp = &g_live; *pp; pp = &(*pp)->next)
+        if ((*pp)->h == h) {
+            live_node *d = *pp;
+            *pp = d->next;
+            free(d);
+            return;
+        }
+}
+
 static int set_msg(char *msg, size_t len, const char *fmt, ...)
 {
     if (msg && len) {
@@ -103,7 +142,7 @@ static mgg_mex_handle *to_handle(const mgg_mex_arg *a, char *msg, size_t len)
         return NULL;
     }
     mgg_mex_handle *h = (mgg_mex_handle *)(uintptr_t)((const uint64_t *)a->data)[0];
-    if (!h || h->signature != MGG_MEX_HANDLE_SIGNATURE || strcmp(h->type_name, kTypeName) != 0) {
+    if (!h || !live_has(h) || h->signature != MGG_MEX_HANDLE_SIGNATURE || strcmp(h->type_name, kTypeName) != 0) {
         set_msg(msg, len, "Handle not valid.");
         return NULL;
     }
@@ -146,6 +185,11 @@ static int create_gridder(int nrhs, const mgg_mex_arg *prhs, uint64_t *handle_ou
     h->signature = MGG_MEX_HANDLE_SIGNATURE;
     memcpy(h->type_name, kTypeName, sizeof kTypeName);
     h->g = g;
+    if (!live_add(h)) {
+        mgg_destroy(g);
+        free(h);
+        return set_msg(msg, len, "%s: out of host memory", E);
+    }
     *handle_out = (uint64_t)(uintptr_t)h;
     return 0;
 }
@@ -250,7 +294,8 @@ static int delete_gridder(int nlhs, int nrhs, const mgg_mex_arg *prhs, char *msg
     mgg_mex_handle *h = to_handle(&prhs[0], msg, len);
     if (!h) return 1;
     const int rc = mgg_destroy(h->g);
-    h->signature = 0;  /* a stale copy of the handle is refused from now on */
+    live_remove(h);  /* a stale copy of the handle is refused from now on */
+    h->signature = 0;
     h->g = NULL;
     free(h);
     if (rc != MGG_OK) return status(rc, "mexDeleteGridder", "mgg_destroy", msg, len);

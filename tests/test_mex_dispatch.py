@@ -129,6 +129,16 @@ def test_invalid_handles_are_refused(ml):
     assert rc == 1 and err == "Handle not valid."
     rc, err, _ = ml("mexMultiGPUForwardProject", ml.handle_arg(0))
     assert rc == 1 and err == "Handle not valid."
+    # a deleted handle is refused, not read
+    rc, err, h = ml.create()
+    assert rc == 0, err
+    assert ml("mexDeleteGridder", ml.handle_arg(h))[0] == 0
+    for entry in ("mexMultiGPUForwardProject", "mexDeleteGridder"):
+        rc, err, _ = ml(entry, ml.handle_arg(h))
+        assert rc == 1 and err == "Handle not valid.", entry
+    # an arbitrary integer is refused without being dereferenced
+    rc, err, _ = ml("mexMultiGPUBackProject", ml.handle_arg(0xDEAD0000BEEF))
+    assert rc == 1 and err == "Handle not valid."
 
 
 def test_set_variables_reaches_each_setter(ml):
