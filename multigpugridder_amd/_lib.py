@@ -91,6 +91,8 @@ _SIGS = {
     "mgg_reconstruct_volume": [_vp],
     "mgg_reset_volume": [_vp],
     "mgg_op_forward_project": [_vp, _ci, _ci, _vp, _ci, _vp, _ci, _cf, _vp, _ci, _cf, _vp],
+    "mgg_op_forward_project_checked": [_vp, _ci, _ci, _vp, _ci, _vp, _ci, _cf, _vp, _ci, _cf, _vp, _vp],
+    "mgg_op_check_volume": [_vp, _ci, _ci, _vp, _ci, _vp, _vp],
     "mgg_op_back_project": [_vp, _vp, _ci, _vp, _ci, _vp, _ci, _cf, _vp, _ci, _cf, _vp],
     "mgg_op_back_project_adjoint": [_vp, _ci, _vp, _ci, _vp, _ci, _cf, _vp, _ci, _cf, _vp],
     "mgg_op_cas_images_to_images": [_vp, _vp, _ci, _ci, _ci, _vp],

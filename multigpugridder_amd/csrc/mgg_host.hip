@@ -60,6 +60,7 @@ struct mgg_gridder {
     int hKerLen = 0;
     // device state
     DevBuf dCasVol, dKer, dPrecomp, dVolTmp, dReal3, dHerm3, dDens;
+    DevBuf dCheck;  // launch_check_volume's flag for dCasVol and dKer
     DevBuf dPartials, dPartialsDens;  // one V^3 partial volume per BP stream, contiguous
     std::vector<float> defaultKer;
     int kerLenOnDevice = 0;
@@ -222,6 +223,12 @@ struct mgg_gridder {
         const int nS = std::max(1, std::min(nStreamsFP, (int)chunks.size()));
         ensure_slots(nS);
         const ProjGeom g = geom();
+        // One check of the CAS volume and the table per call (every stream's
+        // launches come after the device synchronisation below).
+        dCheck.reserve(sizeof(unsigned));
+        check_launch(launch_check_volume(dCasVol.ptr, DT_F32, (size_t)V * V * V, g.ker, g.K, dCheck.as<unsigned>(),
+                                         slots[0]->stream),
+                     "check_volume");
         for (int i = 0; i < nS; i++) {
             Slot &s = *slots[i];
             s.axes.reserve(sizeof(float) * 9 * E);
@@ -238,7 +245,7 @@ struct mgg_gridder {
             HIP_TRY(hipMemcpyAsync(s.axes.ptr, hAxes + (size_t)9 * c.first, sizeof(float) * 9 * c.count,
                                    hipMemcpyHostToDevice, s.stream));
             check_launch(launch_forward_project(dCasVol.ptr, DT_F32, g, s.axes.as<float>(), c.count,
-                                                s.casImgs.as<float>(), s.stream),
+                                                s.casImgs.as<float>(), s.stream, dCheck.as<unsigned>()),
                          "forward_project");
             if (wantCas)
                 HIP_TRY(hipMemcpyAsync(casOut + (size_t)c.first * M * M, s.casImgs.ptr,
@@ -704,9 +711,9 @@ int mgg_reset_volume(mgg_gridder *g) { WITH_G(g->reset_volume()); }
 // ---------------------------------------------------------------------------
 // Device-pointer operators
 // ---------------------------------------------------------------------------
-int mgg_op_forward_project(const void *cas_volume, int cas_dtype, int v, const float *axes, int n_axes,
-                           float *cas_images, int m, float mask_radius, const float *ker, int ker_size,
-                           float ker_hwidth, void *stream)
+int mgg_op_forward_project_checked(const void *cas_volume, int cas_dtype, int v, const float *axes, int n_axes,
+                                   float *cas_images, int m, float mask_radius, const float *ker, int ker_size,
+                                   float ker_hwidth, const unsigned *check, void *stream)
 {
     return guarded([&] {
         if (n_axes == 0) return;
@@ -716,8 +723,33 @@ int mgg_op_forward_project(const void *cas_volume, int cas_dtype, int v, const f
         if (cas_dtype != MGG_DTYPE_F32 && cas_dtype != MGG_DTYPE_BF16 && cas_dtype != MGG_DTYPE_F16)
             raise(MGG_ERR_INVALID_ARGUMENT, "unknown CAS volume dtype %d", cas_dtype);
         ProjGeom g{v, m, mask_radius, ker, ker_size, ker_hwidth};
-        check_launch(launch_forward_project(cas_volume, cas_dtype, g, axes, n_axes, cas_images, as_stream(stream)),
+        check_launch(launch_forward_project(cas_volume, cas_dtype, g, axes, n_axes, cas_images, as_stream(stream),
+                                            check),
                      "forward_project");
+    });
+}
+
+int mgg_op_forward_project(const void *cas_volume, int cas_dtype, int v, const float *axes, int n_axes,
+                           float *cas_images, int m, float mask_radius, const float *ker, int ker_size,
+                           float ker_hwidth, void *stream)
+{
+    return mgg_op_forward_project_checked(cas_volume, cas_dtype, v, axes, n_axes, cas_images, m, mask_radius, ker,
+                                          ker_size, ker_hwidth, nullptr, stream);
+}
+
+int mgg_op_check_volume(const void *cas_volume, int cas_dtype, int v, const float *ker, int ker_size,
+                        unsigned *check, void *stream)
+{
+    return guarded([&] {
+        need(cas_volume, "cas_volume"); need(ker, "ker"); need(check, "check");
+        if (v <= 0) raise(MGG_ERR_INVALID_ARGUMENT, "volume size must be positive, got %d", v);
+        if (ker_size <= 0 || ker_size > kMaxKerSize)
+            raise(MGG_ERR_INVALID_ARGUMENT, "kernel table size %d outside 1..%d", ker_size, kMaxKerSize);
+        if (cas_dtype != MGG_DTYPE_F32 && cas_dtype != MGG_DTYPE_BF16 && cas_dtype != MGG_DTYPE_F16)
+            raise(MGG_ERR_INVALID_ARGUMENT, "unknown CAS volume dtype %d", cas_dtype);
+        check_launch(launch_check_volume(cas_volume, cas_dtype, (size_t)v * v * v, ker, ker_size, check,
+                                         as_stream(stream)),
+                     "check_volume");
     });
 }
 

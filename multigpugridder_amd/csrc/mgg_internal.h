@@ -28,9 +28,17 @@ struct ProjGeom {
 // ---- projectors --------------------------------------------------------
 // Forward projection, reference semantics: casImgs[a] (M*M) is fully written
 // (0 outside the mask disc).
+// With the 501-entry table and check pointing at a flag launch_check_volume
+// set to 1 (same stream, after the volume's last change), the gather reads
+// only each axis' four taps that can have nonzero weight (same sums, smaller
+// voxel boxes); check == nullptr gathers the full 5x5x5 stencils.
 hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
                                   const float *axes, int nAxes, float *casImgs,
-                                  hipStream_t s);
+                                  hipStream_t s, const unsigned *check = nullptr);
+// *check = 1 when the table has 501 finite entries with both ends 0 and the
+// volume's n values (storage dtype) are all finite, else 0 (two launches on s).
+hipError_t launch_check_volume(const void *vol, int dtype, size_t n, const float *ker, int K, unsigned *check,
+                               hipStream_t s);
 // Back projection, reference semantics (voxel-owner gather, no atomics).
 // volAcc (V^3) and, if non-null, densAcc (V^3, the plane density = back
 // projection of all-ones CAS images) are accumulated into.
@@ -47,6 +55,7 @@ struct FPSegment {
     const float *axes;
     float *out;
     const float *ker;
+    const unsigned *check;  // launch_check_volume's flag for vol and ker, or nullptr
     int V, M, nAxes, maskD2, tilesPerRow, tilesPerImg;
     unsigned workStart;
 };
@@ -61,7 +70,7 @@ bool fp_segment_supported(const ProjGeom &g);
 // Fills sg for nAxes directions of the volume vol (FP32) and returns its work
 // items (tiles); the caller sets the next segment's workStart past them.
 unsigned fp_segment(FPSegment &sg, const void *vol, const ProjGeom &g, const float *axes, int nAxes,
-                    float *casImgs, unsigned workStart);
+                    float *casImgs, unsigned workStart, const unsigned *check = nullptr);
 hipError_t launch_forward_project_multi(const FPMultiArgs &m, hipStream_t s);
 // The fast back projection walks bricks in a centre-out order; these give the
 // table length (0 when the generic kernel is used) and fill it (host memory).

@@ -26,6 +26,7 @@ namespace mgg {
 namespace {
 
 typedef float f4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u3 __attribute__((ext_vector_type(3)));
 
 constexpr int kThreads = 256;
@@ -141,6 +142,7 @@ struct FPArgs {
     int convW, tilesPerRow, tilesPerImg;
     unsigned perXcd, totalWork;
     int maskD2;  // fp_lds_kernel: largest integer d2 with sqrtf((float)d2) <= maskR (-1: none)
+    const unsigned *check;  // fp_lds_kernel with the 501-entry table: launch_check_volume's flag, or null
 };
 
 constexpr int KFP_FAST = 501;  // the table the mixed-volume launch requires (kerScale 125, kerCenter 250)
@@ -324,28 +326,28 @@ __device__ __forceinline__ void load4<VolLoad<DT_F16>>(const uint16_t *p, float 
     v[2] = VolLoad<DT_F16>::h2f(a.y); v[3] = VolLoad<DT_F16>::h2f(a.y >> 16);
 }
 
-template <int DT, bool FAST>
-__device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__restrict__ sker, float *__restrict__ box,
-                                            unsigned work, int ln)
+// Tile body.  W4 = false: every pixel gathers its full 5x5x5 stencil from the
+// box of all its stencils.  W4 = true (501-entry table whose end entries are 0,
+// volume known to be finite): along each axis the reference's taps X-2..X+2
+// (X = roundf(x)) have r = tap - x clamped to [-2, 2], and the tap with r <= -2
+// reads table entry 0 and the one with r >= 2 entry 500 (fma(-2, 125, 250) = 0
+// and fma(2, 125, 250) = 500 exactly).  If x >= X, then X - 2 - x <= -2; if
+// x < X, then X + 2 - x > 2: one of the two end taps always has weight 0, and
+// the others are the four taps from Xs = X - 2 + (x >= X) = floor(x) - 1.  The
+// zero taps add fma(0, v, s) = s for finite v, so gathering the 4x4x4 window
+// gives the same sums bit for bit, and the box shrinks to the voxels the
+// windows touch: [floor(min) - 1, floor(max) + 2] per axis instead of
+// [floor(min) - 2, ceil(max) + 2], about 40% fewer voxels to stage.
+template <int DT, bool FAST, bool W4>
+__device__ __forceinline__ void fp_lds_tile_body(const FPArgs &p, const float *__restrict__ sker,
+                                                 float *__restrict__ box, int a, int i0, int j0, int ln,
+                                                 bool active, bool pix, float *out)
 {
     using L = VolLoad<DT>;
     using T = typename L::T;
-    const int a = (int)(work / (unsigned)p.tilesPerImg);
-    const int tile = (int)(work - (unsigned)a * (unsigned)p.tilesPerImg);
-    const int ty = tile / p.tilesPerRow, tx = tile - ty * p.tilesPerRow;
-    const int i0 = tx * FPT, j0 = ty * FPT;
+    constexpr int NW = W4 ? 4 : 5;  // taps per axis
     const int i = i0 + (ln & 7), j = j0 + (ln >> 3);
     const int M = p.M, V = p.V, ci = M / 2, cv = V / 2;
-    const bool pix = i < M && j < M;
-
-    // sqrtf(d2) <= maskR for the exact integer d2, as one integer compare
-    // (sqrtf is monotone; the host finds the largest passing d2).
-    const bool active = pix && (i - ci) * (i - ci) + (j - ci) * (j - ci) <= p.maskD2;
-    float *out = p.out + (size_t)a * M * M + (size_t)j * M + i;
-    if (__ballot(active) == 0ull) {
-        if (pix) *out = 0.f;
-        return;
-    }
     const float *ax = p.axes + (size_t)9 * a;
     const float u0 = ax[0], u1 = ax[1], u2 = ax[2];
     const float v0 = ax[3], v1 = ax[4], v2 = ax[5];
@@ -354,21 +356,25 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     const float y = ref_sum2c(u1, fi, v1, fj, (float)cv);
     const float z = ref_sum2c(u2, fi, v2, fj, (float)cv);
     const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
+    // first tap of each axis' window
+    const int Xs = X - 2 + ((W4 && x >= (float)X) ? 1 : 0);
+    const int Ys = Y - 2 + ((W4 && y >= (float)Y) ? 1 : 0);
+    const int Zs = Z - 2 + ((W4 && z >= (float)Z) ? 1 : 0);
 
     const long long V3 = (long long)V * V * V;
     const T *vol = (const T *)p.vol;
 
-    float wi[5], wj[5], wk[5];
+    float wi[NW], wj[NW], wk[NW];
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
+    for (int t = 0; t < NW; t++) {
         if (FAST) {
-            wi[t] = kb_fast_fp(p.ker, (float)(X - 2 + t) - x);
-            wj[t] = kb_fast_fp(p.ker, (float)(Y - 2 + t) - y);
-            wk[t] = kb_fast_fp(p.ker, (float)(Z - 2 + t) - z);
+            wi[t] = kb_fast_fp(p.ker, (float)(Xs + t) - x);
+            wj[t] = kb_fast_fp(p.ker, (float)(Ys + t) - y);
+            wk[t] = kb_fast_fp(p.ker, (float)(Zs + t) - z);
         } else {
-            wi[t] = sker[kb_index((float)(X - 2 + t) - x, p.fW, p.kerScale, p.kerCenter, p.K)];
-            wj[t] = sker[kb_index((float)(Y - 2 + t) - y, p.fW, p.kerScale, p.kerCenter, p.K)];
-            wk[t] = sker[kb_index((float)(Z - 2 + t) - z, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wi[t] = sker[kb_index((float)(Xs + t) - x, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wj[t] = sker[kb_index((float)(Ys + t) - y, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wk[t] = sker[kb_index((float)(Zs + t) - z, p.fW, p.kerScale, p.kerCenter, p.K)];
         }
     }
 
@@ -378,16 +384,16 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     float acc = 0.f;
     auto stageAndGather = [&](int rlo, int rhi, bool mine, auto splitT) -> bool {
         // Box from the corners: positions are affine in (i, j), so every
-        // pixel's rounded coordinate lies in [floor(min corner), ceil(max
-        // corner)] (the 1e-3 margin covers FP32 evaluation error), widened by
-        // the stencil.
+        // pixel's coordinate lies in [min corner, max corner] (the 1e-3 margin
+        // covers FP32 evaluation error), widened by the window.
         const float ai = (float)(i0 - ci), bi = (float)(i0 + FPT - 1 - ci);
         const float aj = (float)(j0 + rlo - ci), bj = (float)(j0 + rhi - ci);
         auto lo = [&](float uu, float vv) {
-            return (int)floorf(fminf(uu * ai, uu * bi) + fminf(vv * aj, vv * bj) + (float)cv - 1e-3f) - 2;
+            return (int)floorf(fminf(uu * ai, uu * bi) + fminf(vv * aj, vv * bj) + (float)cv - 1e-3f) - (W4 ? 1 : 2);
         };
         auto hi = [&](float uu, float vv) {
-            return (int)ceilf(fmaxf(uu * ai, uu * bi) + fmaxf(vv * aj, vv * bj) + (float)cv + 1e-3f) + 2;
+            const float e = fmaxf(uu * ai, uu * bi) + fmaxf(vv * aj, vv * bj) + (float)cv + 1e-3f;
+            return (int)(W4 ? floorf(e) : ceilf(e)) + 2;
         };
         const int bx0 = __builtin_amdgcn_readfirstlane(lo(u0, v0));
         const int by0 = __builtin_amdgcn_readfirstlane(lo(u1, v1));
@@ -395,10 +401,12 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         const int Dx = __builtin_amdgcn_readfirstlane(hi(u0, v0)) - bx0 + 1;
         const int Dy = __builtin_amdgcn_readfirstlane(hi(u1, v1)) - by0 + 1;
         const int Dz = __builtin_amdgcn_readfirstlane(hi(u2, v2)) - bz0 + 1;
-        if (!(Dx >= 5 && Dy >= 5 && Dz >= 5 && Dx <= 64 && Dy <= 64 && Dz <= 64 &&
+        if (!(Dx >= NW && Dy >= NW && Dz >= NW && Dx <= 64 && Dy <= 64 && Dz <= 64 &&
               V3 * (long long)sizeof(T) < (1LL << 32)))
             return false;
-        const int segs = (Dx + 3) >> 2;  // 16-byte segments per box row
+        // 16-byte segments per box row: enough that the 8-wide read of a window
+        // (below) stays inside the row's staged segments.
+        const int segs = (Dx + 8 - NW) >> 2;
         // LDS pitches (floats), multiples of 4 (16-byte rows).  When the box
         // still fits, the row and plane pitches are padded to an odd number
         // of 16-byte units so that the rows and planes a tile's pixels read
@@ -429,15 +437,16 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         // needs a check.  lin grows with ez, ey and the segment, so the range
         // is spanned by the first and last segment of the box.
         const long long linLo = (long long)bz0 * VV + (long long)by0 * V + bx0;
-        const long long linHi = (long long)(bz0 + Dz - 1) * VV + (long long)(by0 + Dy - 1) * V + bx0 + RP - 1;
+        const long long linHi = (long long)(bz0 + Dz - 1) * VV + (long long)(by0 + Dy - 1) * V + bx0 + 4 * segs - 1;
         const bool boxIn = linLo >= 0 && linHi < V3;
         // Six planes per batch: all loads of a batch are in flight before the
         // first LDS store waits on them (2, 4 and 6 planes: 19.9, 18.5 and
-        // 18.3 ms per 10,000 directions at N=128; 8 needs 129 VGPRs, three
-        // waves per SIMD instead of four, 22.0 ms).  The checked and
-        // unchecked variants are separate loops (boxIn is wave-uniform); the
-        // unchecked one addresses the volume by a 32-bit byte offset from its
-        // base (V^3 * sizeof(T) < 2^32 is required above).
+        // 18.3 ms per 10,000 directions at N=128 with the 5-wide windows; 8
+        // needs more than 128 VGPRs, three waves per SIMD instead of four,
+        // 22.0 ms).  The checked and unchecked variants are separate loops
+        // (boxIn is wave-uniform); the unchecked one addresses the volume by a
+        // 32-bit byte offset from its base (V^3 * sizeof(T) < 2^32 is
+        // required above).
         constexpr int SB = 6;
         const int lStep = G * VV, dStep = G * PP;
         for (int c = 0; c < nLanePass; c++) {
@@ -484,39 +493,40 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
                 stage(std::false_type{});
             }
         }
-        // 16-byte gather: each 5-tap row is read as the two aligned 16-byte
+        // 16-byte gather: each NW-tap row is read as the two aligned 16-byte
         // units that contain it, [xo - o, xo - o + 8) with o = xo & 3, and the
-        // x weights are shifted by o with zeros in the three unused slots.
-        // Since xo <= Dx - 5, xo - o + 8 <= 4 * ceil(Dx / 4) = 4 * segs, so
-        // the read stays inside the row's staged segments.  The unused values
-        // are staged volume values times 0, which leaves every sum
-        // bit-identical to the 5-tap read when they are finite; a non-finite
-        // one makes the pixel's sum NaN (0 * NaN and 0 * Inf are NaN).  A
-        // pixel whose sum is not finite is therefore gathered again with the
-        // 5-tap read, so NaN and Inf reach exactly the pixels whose stencil
-        // holds them, as in the reference.
+        // x weights are shifted by o with zeros in the unused slots.  Since
+        // xo <= Dx - NW, xo - o + 8 <= 4 * floor((Dx - NW) / 4) + 8, which
+        // is at most 4 * segs, so the read stays inside the row's staged
+        // segments.  The unused values are staged volume values times 0,
+        // which leaves every sum bit-identical to the NW-tap read when they
+        // are finite.  With the 5-wide window a non-finite one makes the
+        // pixel's sum NaN (0 * NaN and 0 * Inf are NaN), so a pixel whose sum
+        // is not finite is gathered again with the 5-tap read, and NaN and Inf
+        // reach exactly the pixels whose stencil holds them, as in the
+        // reference; the 4-wide window runs only on finite volumes.
         wave_lds_sync_fp();
         // x weights shifted by o (zeros in the unused slots) times the y weights.
-        auto shifted = [&](const float (&wx)[5], const float (&wy)[5], int o, float (&w8)[8][5]) {
+        auto shifted = [&](const float (&wx)[NW], const float (&wy)[NW], int o, float (&w8)[8][NW]) {
             float ws[8];
 #pragma unroll
             for (int t = 0; t < 8; t++) {
                 float w = 0.f;
 #pragma unroll
                 for (int sft = 0; sft < 4; sft++)
-                    if (t - sft >= 0 && t - sft < 5) w = (o == sft) ? wx[t - sft] : w;
+                    if (t - sft >= 0 && t - sft < NW) w = (o == sft) ? wx[t - sft] : w;
                 ws[t] = w;
             }
 #pragma unroll
             for (int t = 0; t < 8; t++)
 #pragma unroll
-                for (int bb = 0; bb < 5; bb++) w8[t][bb] = ws[t] * wy[bb];
+                for (int bb = 0; bb < NW; bb++) w8[t][bb] = ws[t] * wy[bb];
         };
-        // One z plane of a stencil: five rows of the 16-byte read.
-        auto planeSum = [&](const float *bq, const float (&w8)[8][5], int cc) {
+        // One z plane of a window: NW rows of the 16-byte read.
+        auto planeSum = [&](const float *bq, const float (&w8)[8][NW], int cc) {
             float sc = 0.f;
 #pragma unroll
-            for (int bb = 0; bb < 5; bb++) {
+            for (int bb = 0; bb < NW; bb++) {
                 const float *rp = bq + cc * PP + bb * RP;
                 const f4 q0 = *(const f4 *)rp, q1 = *(const f4 *)(rp + 4);
                 sc = fmaf(w8[0][bb], q0.x, sc);
@@ -534,57 +544,59 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         bool gathered = false;
         if constexpr (!decltype(splitT)::value) {
             if (active && mine) {
-                const int xo = X - 2 - bx0, o = xo & 3;
-                const float *bq = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (xo - o);
-                float w8[8][5];
+                const int xo = Xs - bx0, o = xo & 3;
+                const float *bq = box + (Zs - bz0) * PP + (Ys - by0) * RP + (xo - o);
+                float w8[8][NW];
                 shifted(wi, wj, o, w8);
 #pragma unroll
-                for (int cc = 0; cc < 5; cc++) res = fmaf(wk[cc], planeSum(bq, w8, cc), res);
+                for (int cc = 0; cc < NW; cc++) res = fmaf(wk[cc], planeSum(bq, w8, cc), res);
                 gathered = true;
             }
         } else {
             // Half tile: the lanes of the other half (lane ^ 32; their own
-            // pixels are not in this box) gather planes 3 and 4 of their
-            // partner's stencil while the owner gathers planes 0-2, so the wave
-            // reads 15 rows per pixel instead of 25; the owner adds the parts.
-            const int pX = __shfl_xor(X, 32), pY = __shfl_xor(Y, 32), pZ = __shfl_xor(Z, 32);
+            // pixels are not in this box) gather the last two planes of their
+            // partner's window while the owner gathers the first NW - 2, so
+            // the wave reads 3 * NW rows per pixel instead of NW * NW; the
+            // owner adds the parts.
+            constexpr int H = NW - 2;  // planes the owner gathers
+            const int pX = __shfl_xor(Xs, 32), pY = __shfl_xor(Ys, 32), pZ = __shfl_xor(Zs, 32);
             const int pAct = __shfl_xor(active ? 1 : 0, 32);
-            float gwi[5], gwj[5];
+            float gwi[NW], gwj[NW];
 #pragma unroll
-            for (int t = 0; t < 5; t++) {
+            for (int t = 0; t < NW; t++) {
                 const float a5 = __shfl_xor(wi[t], 32), b5 = __shfl_xor(wj[t], 32);
                 gwi[t] = mine ? wi[t] : a5;
                 gwj[t] = mine ? wj[t] : b5;
             }
-            const float pk3 = __shfl_xor(wk[3], 32), pk4 = __shfl_xor(wk[4], 32);
-            const int gX = mine ? X : pX, gY = mine ? Y : pY, gZ = mine ? Z : pZ + 3;
-            const float k0 = mine ? wk[0] : pk3, k1 = mine ? wk[1] : pk4;
+            const float pkA = __shfl_xor(wk[H], 32), pkB = __shfl_xor(wk[H + 1], 32);
+            const int gX = mine ? Xs : pX, gY = mine ? Ys : pY, gZ = mine ? Zs : pZ + H;
+            const float k0 = mine ? wk[0] : pkA, k1 = mine ? wk[1] : pkB;
             float part = 0.f;
             if (mine ? active : pAct != 0) {
-                const int xo = gX - 2 - bx0, o = xo & 3;
-                const float *bq = box + (gZ - 2 - bz0) * PP + (gY - 2 - by0) * RP + (xo - o);
-                float w8[8][5];
+                const int xo = gX - bx0, o = xo & 3;
+                const float *bq = box + (gZ - bz0) * PP + (gY - by0) * RP + (xo - o);
+                float w8[8][NW];
                 shifted(gwi, gwj, o, w8);
                 part = fmaf(k0, planeSum(bq, w8, 0), part);
                 part = fmaf(k1, planeSum(bq, w8, 1), part);
-                if (mine) part = fmaf(wk[2], planeSum(bq, w8, 2), part);
+                if (!W4 && mine) part = fmaf(wk[2], planeSum(bq, w8, 2), part);
             }
             const float other = __shfl_xor(part, 32);
             res = part + other;
             gathered = active && mine;
         }
         if (gathered) {
-            if (!__builtin_isfinite(res)) {
+            if (!W4 && !__builtin_isfinite(res)) {
                 const float *bp = box + (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (X - 2 - bx0);
                 res = 0.f;
 #pragma unroll
-                for (int cc = 0; cc < 5; cc++) {
+                for (int cc = 0; cc < NW; cc++) {
                     float sc = 0.f;
 #pragma unroll
-                    for (int bb = 0; bb < 5; bb++) {
+                    for (int bb = 0; bb < NW; bb++) {
                         const float *rp = bp + cc * PP + bb * RP;
 #pragma unroll
-                        for (int aa = 0; aa < 5; aa++) sc = fmaf(wi[aa] * wj[bb], rp[aa], sc);
+                        for (int aa = 0; aa < NW; aa++) sc = fmaf(wi[aa] * wj[bb], rp[aa], sc);
                     }
                     res = fmaf(wk[cc], sc, res);
                 }
@@ -595,9 +607,9 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         return true;
     };
 
-    // The whole 8x8 tile; if its box is too large (27% of the tiles for the
-    // grid directions at N=128), its two 8x4 halves one after the other; a
-    // half that still does not fit gathers from global memory.
+    // The whole 8x8 tile; if its box is too large, its two 8x4 halves one
+    // after the other; a half that still does not fit gathers from global
+    // memory.
     bool done = stageAndGather(0, FPT - 1, true, std::false_type{});
     if (!done) {
         const bool top = (ln >> 3) < FPT / 2;
@@ -607,19 +619,52 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
     }
     if (!done && active) {
 #pragma unroll
-        for (int cc = 0; cc < 5; cc++) {
+        for (int cc = 0; cc < NW; cc++) {
             float sc = 0.f;
 #pragma unroll
-            for (int bb = 0; bb < 5; bb++)
+            for (int bb = 0; bb < NW; bb++)
 #pragma unroll
-                for (int aa = 0; aa < 5; aa++) {
-                    const long long lin = (long long)(Z - 2 + cc) * V * V + (long long)(Y - 2 + bb) * V + (X - 2 + aa);
+                for (int aa = 0; aa < NW; aa++) {
+                    const long long lin = (long long)(Zs + cc) * V * V + (long long)(Ys + bb) * V + (Xs + aa);
                     if (lin < V3 && lin >= 0) sc = fmaf(wi[aa] * wj[bb], L::one(vol + lin), sc);
                 }
             acc = fmaf(wk[cc], sc, acc);
         }
     }
     if (pix) *out = active ? acc : 0.f;
+}
+
+// One 8x8 tile of one direction; W4 (only with the 501-entry table) selects
+// the 4-wide windows.
+template <int DT, bool FAST, bool W4>
+__device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__restrict__ sker, float *__restrict__ box,
+                                            unsigned work, int ln)
+{
+    const int a = (int)(work / (unsigned)p.tilesPerImg);
+    const int tile = (int)(work - (unsigned)a * (unsigned)p.tilesPerImg);
+    const int ty = tile / p.tilesPerRow, tx = tile - ty * p.tilesPerRow;
+    const int i0 = tx * FPT, j0 = ty * FPT;
+    const int i = i0 + (ln & 7), j = j0 + (ln >> 3);
+    const int M = p.M, ci = M / 2;
+    const bool pix = i < M && j < M;
+
+    // sqrtf(d2) <= maskR for the exact integer d2, as one integer compare
+    // (sqrtf is monotone; the host finds the largest passing d2).
+    const bool active = pix && (i - ci) * (i - ci) + (j - ci) * (j - ci) <= p.maskD2;
+    float *out = p.out + (size_t)a * M * M + (size_t)j * M + i;
+    if (__ballot(active) == 0ull) {
+        if (pix) *out = 0.f;
+        return;
+    }
+    fp_lds_tile_body<DT, FAST, FAST && W4>(p, sker, box, a, i0, j0, ln, active, pix, out);
+}
+
+// The 4-wide windows need a finite volume and a finite 501-entry table whose
+// end entries are 0 (the reference's table sets them so): launch_check_volume
+// writes *check = 1 when they hold.  Wave-uniform.
+__device__ __forceinline__ bool fp_win4(const unsigned *check)
+{
+    return check != nullptr && __builtin_amdgcn_readfirstlane(*check) == 1u;
 }
 
 template <int DT, bool FAST>
@@ -635,8 +680,15 @@ __global__ __launch_bounds__(kThreads) void fp_lds_kernel(FPArgs p)
     // Each wave takes work items wv + 4*blockIdx.x, + 4*gridDim.x, ...  (wave
     // uniform, so the work index and its divisions stay scalar)
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
-    for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
-        fp_lds_tile<DT, FAST>(p, sker, sbox[wv], work, ln);
+    // The window width is chosen once, outside the tile loop, so that each
+    // loop is register-allocated on its own.
+    if (FAST && fp_win4(p.check)) {
+        for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
+            fp_lds_tile<DT, FAST, true>(p, sker, sbox[wv], work, ln);
+    } else {
+        for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
+            fp_lds_tile<DT, FAST, false>(p, sker, sbox[wv], work, ln);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -715,8 +767,9 @@ __global__ __launch_bounds__(kThreads) void fp_lds_multi_kernel(FPMultiArgs m)
         a.V = sg.V; a.M = sg.M; a.K = KFP_FAST; a.nAxes = sg.nAxes;
         a.maskR = 0.f; a.kerScale = 125.0f; a.kerCenter = 250.0f; a.fW = 2.0f;
         a.convW = 2; a.tilesPerRow = sg.tilesPerRow; a.tilesPerImg = sg.tilesPerImg;
-        a.perXcd = 0; a.totalWork = 0; a.maskD2 = sg.maskD2;
-        fp_lds_tile<DT, true>(a, nullptr, sbox[wv], work - sg.workStart, ln);
+        a.perXcd = 0; a.totalWork = 0; a.maskD2 = sg.maskD2; a.check = sg.check;
+        if (fp_win4(sg.check)) fp_lds_tile<DT, true, true>(a, nullptr, sbox[wv], work - sg.workStart, ln);
+        else fp_lds_tile<DT, true, false>(a, nullptr, sbox[wv], work - sg.workStart, ln);
     }
 }
 
@@ -2270,6 +2323,49 @@ __global__ void k_fill(float *__restrict__ p, float v, size_t n)
 }
 
 // ---------------------------------------------------------------------------
+// Check behind the forward projector's 4-wide windows: *check = 1 when the
+// table has the 501 entries, all finite, with both end entries 0, and every
+// value of the volume is finite; else 0.  k_check_table (one workgroup) writes
+// the table's verdict, then k_check_volume clears it on any non-finite voxel
+// (all finding workgroups store the same 0).
+// ---------------------------------------------------------------------------
+__global__ void k_check_table(const float *__restrict__ ker, int K, unsigned *__restrict__ check)
+{
+    bool ok = true;
+    for (int t = threadIdx.x; t < K; t += blockDim.x) ok = ok && __builtin_isfinite(ker[t]);
+    ok = __syncthreads_and(ok);
+    if (threadIdx.x == 0) *check = (ok && K == KFP_FAST && ker[0] == 0.f && ker[K - 1] == 0.f) ? 1u : 0u;
+}
+
+// Non-finite = exponent field all ones (FP32 0x7f800000, BF16 0x7f80, FP16 0x7c00).
+template <int DT>
+__global__ void k_check_volume(const void *__restrict__ vol, size_t n, unsigned *__restrict__ check)
+{
+    const size_t t0 = (size_t)blockIdx.x * blockDim.x + threadIdx.x, st = (size_t)gridDim.x * blockDim.x;
+    bool bad = false;
+    if (DT == DT_F32) {
+        const unsigned *w = (const unsigned *)vol;
+        size_t e0 = 0;
+        if (((uintptr_t)w & 15u) == 0) {  // 16-byte loads over the aligned body
+            const size_t n4 = n / 4;
+            const u4 *w4 = (const u4 *)w;
+            for (size_t e = t0; e < n4; e += st) {
+                const u4 q = w4[e];
+                bad |= ((q.x & 0x7f800000u) == 0x7f800000u) | ((q.y & 0x7f800000u) == 0x7f800000u) |
+                       ((q.z & 0x7f800000u) == 0x7f800000u) | ((q.w & 0x7f800000u) == 0x7f800000u);
+            }
+            e0 = n4 * 4;
+        }
+        for (size_t e = e0 + t0; e < n; e += st) bad |= (w[e] & 0x7f800000u) == 0x7f800000u;
+    } else {
+        const uint16_t *h = (const uint16_t *)vol;
+        const unsigned m = DT == DT_BF16 ? 0x7f80u : 0x7c00u;
+        for (size_t e = t0; e < n; e += st) bad |= ((unsigned)h[e] & m) == m;
+    }
+    if (__syncthreads_or(bad) && threadIdx.x == 0) *check = 0u;
+}
+
+// ---------------------------------------------------------------------------
 // Standalone filters (Filters/*.cu).  Same element semantics as the reference
 // kernels; grid-stride 1-D launches instead of one thread per (x, y) column.
 // ---------------------------------------------------------------------------
@@ -2481,11 +2577,12 @@ inline void proj_consts(const ProjGeom &g, float &kerScale, float &kerCenter, fl
 // Launchers
 // ===========================================================================
 hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
-                                  const float *axes, int nAxes, float *casImgs, hipStream_t s)
+                                  const float *axes, int nAxes, float *casImgs, hipStream_t s,
+                                  const unsigned *check)
 {
     if (nAxes <= 0) return hipSuccess;
     FPArgs a;
-    a.vol = vol; a.axes = axes; a.out = casImgs; a.ker = g.ker;
+    a.vol = vol; a.axes = axes; a.out = casImgs; a.ker = g.ker; a.check = nullptr;
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
     a.maskR = ref_mask_radius(g.maskRadius);
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
@@ -2493,6 +2590,7 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
     if (w2) {
         // LDS-staged kernel: 8x8 tiles, one per wave, 4 per workgroup.
         const bool fast = a.kerScale == 125.0f && a.kerCenter == 250.0f && g.K == 501;
+        a.check = fast ? check : nullptr;
         a.tilesPerRow = (g.M + FPT - 1) / FPT;
         a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
         a.maskD2 = mask_d2(a.maskR, g.V);
@@ -2549,9 +2647,9 @@ bool fp_segment_supported(const ProjGeom &g)
 }
 
 unsigned fp_segment(FPSegment &sg, const void *vol, const ProjGeom &g, const float *axes, int nAxes, float *casImgs,
-                    unsigned workStart)
+                    unsigned workStart, const unsigned *check)
 {
-    sg.vol = vol; sg.axes = axes; sg.out = casImgs; sg.ker = g.ker;
+    sg.vol = vol; sg.axes = axes; sg.out = casImgs; sg.ker = g.ker; sg.check = check;
     sg.V = g.V; sg.M = g.M; sg.nAxes = nAxes;
     sg.maskD2 = mask_d2(ref_mask_radius(g.maskRadius), g.V);
     sg.tilesPerRow = (g.M + FPT - 1) / FPT;
@@ -2619,7 +2717,7 @@ hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const f
 {
     if (nAxes <= 0) return hipSuccess;
     FPArgs a;
-    a.vol = nullptr; a.axes = axes; a.out = nullptr; a.ker = g.ker;
+    a.vol = nullptr; a.axes = axes; a.out = nullptr; a.ker = g.ker; a.check = nullptr;
     a.V = g.V; a.M = g.M; a.K = g.K; a.nAxes = nAxes;
     a.maskR = ref_mask_radius(g.maskRadius);
     proj_consts(g, a.kerScale, a.kerCenter, a.fW, a.convW);
@@ -2846,6 +2944,20 @@ hipError_t launch_convert_volume(void *dst, int dtype, const float *src, size_t 
         hipLaunchKernelGGL(k_to_f16, dim3(grid_for(n)), dim3(kThreads), 0, s, (uint16_t *)dst, src, n);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_check_volume(const void *vol, int dtype, size_t n, const float *ker, int K, unsigned *check,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(k_check_table, dim3(1), dim3(kThreads), 0, s, ker, K, check);
+    const dim3 grid(std::min(grid_for(n), 4096u)), block(kThreads);
+    switch (dtype) {
+    case DT_F32: hipLaunchKernelGGL(k_check_volume<DT_F32>, grid, block, 0, s, vol, n, check); break;
+    case DT_BF16: hipLaunchKernelGGL(k_check_volume<DT_BF16>, grid, block, 0, s, vol, n, check); break;
+    case DT_F16: hipLaunchKernelGGL(k_check_volume<DT_F16>, grid, block, 0, s, vol, n, check); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

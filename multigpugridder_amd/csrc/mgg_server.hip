@@ -144,7 +144,8 @@ struct Volume {
     int id = 0;
     int N = 0, M = 0, V = 0, pad = 0;
     float maskRadius = 0.f;
-    float *cas = nullptr;  // arena
+    float *cas = nullptr;        // arena
+    unsigned *check = nullptr;   // after cas: launch_check_volume's flag for cas and the table
     size_t bytes = 0;
     int users = 0;         // requests submitted and not finished
 };
@@ -412,7 +413,7 @@ struct mgg_server {
                     const Piece &pc = pieces[q];
                     const Volume &v = *b.segs[pc.seg].req->vol;
                     work += fp_segment(ma.seg[ma.nSegs++], v.cas, geom(v), dAxes + (size_t)9 * pc.lo, pc.hi - pc.lo,
-                                       dCas + pc.cas, work);
+                                       dCas + pc.cas, work, v.check);
                 }
                 ma.totalWork = work;
                 check_launch(launch_forward_project_multi(ma, p.stream), "forward_project_multi");
@@ -890,7 +891,7 @@ int mgg_server_add_volume(mgg_server *s, const float *volume, int volume_size, f
         v->maskRadius = ((float)volume_size * interp_factor) / 2 - 1;  // AbstractGridder.cpp:13
         DeviceGuard dg(s->device);
         const size_t M = v->M, N = v->N, V3 = (size_t)v->V * v->V * v->V;
-        v->bytes = align_up(sizeof(float) * V3);
+        v->bytes = align_up(sizeof(float) * V3) + align_up(sizeof(unsigned));
         const size_t tmp = align_up(sizeof(float) * N * N * N) + align_up(sizeof(float) * M * M * M) +
                            align_up(sizeof(float2) * M * M * (M / 2 + 1));
         void *casMem = nullptr, *tmpMem = nullptr;
@@ -915,6 +916,11 @@ int mgg_server_add_volume(mgg_server *s, const float *volume, int volume_size, f
             HIP_TRY(hipMemcpyAsync(dVol, volume, sizeof(float) * N * N * N, hipMemcpyHostToDevice, s->adminStream));
             chain_volume_to_cas(dVol, v->N, (float *)casMem, v->M, v->pad, s->precomp_for(v->M), dReal3, dHerm3,
                                 s->adminStream);
+            // The volume is immutable while resident: one check serves every
+            // forward batch on it.
+            unsigned *check = (unsigned *)((char *)casMem + align_up(sizeof(float) * V3));
+            check_launch(launch_check_volume(casMem, DT_F32, V3, s->dKer.as<float>(), 501, check, s->adminStream),
+                         "check_volume");
             HIP_TRY(hipStreamSynchronize(s->adminStream));
         } catch (const Failure &f) {
             code = f.code;
@@ -926,6 +932,7 @@ int mgg_server_add_volume(mgg_server *s, const float *volume, int volume_size, f
             throw Failure{code};
         }
         v->cas = (float *)casMem;
+        v->check = (unsigned *)((char *)casMem + align_up(sizeof(float) * V3));
         v->id = s->nextVolume++;
         s->residentBytes += v->bytes;
         *volume_id = v->id;

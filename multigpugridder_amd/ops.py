@@ -92,12 +92,40 @@ def default_mask_radius(m: int) -> float:
     return float(np.float32(m) / 2 - 1)
 
 
+def check_volume(cas_volume: torch.Tensor, ker: torch.Tensor | None = None,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    """One-element int32 device flag: 1 when ker is the 501-entry table with
+    finite entries and zero ends and every value of cas_volume is finite.
+
+    With it forward_project skips each stencil axis' end tap whose weight is
+    table entry 0 or 500 (both 0): same results, smaller staged voxel boxes.
+    The flag is valid for this volume and table until either changes."""
+    if cas_volume.dtype not in _DTYPES:
+        raise ValueError(f"unsupported CAS volume dtype {cas_volume.dtype}")
+    v = _cube_edge(cas_volume, "cas_volume")
+    if out is not None:
+        _need_numel(out, 1, "out")
+    _same_device(("cas_volume", cas_volume), ("ker", ker), ("out", out))
+    _check(cas_volume, "cas_volume", None)
+    ker = kb_table(cas_volume.device) if ker is None else ker
+    _check(ker, "ker")
+    if out is None:
+        out = torch.empty(1, device=cas_volume.device, dtype=torch.int32)
+    _check(out, "out", torch.int32)
+    _lib.call("mgg_op_check_volume", _ptr(cas_volume), _DTYPES[cas_volume.dtype], v, _ptr(ker), ker.numel(),
+              _ptr(out), _stream())
+    return out
+
+
 def forward_project(cas_volume: torch.Tensor, axes: torch.Tensor, m: int, mask_radius=None,
                     ker: torch.Tensor | None = None, hwidth: float = 2.0,
-                    out: torch.Tensor | None = None) -> torch.Tensor:
+                    out: torch.Tensor | None = None, check=None) -> torch.Tensor:
     """Reference-semantics forward projection -> (P, m, m) FP32 CAS images.
 
-    cas_volume may be float32, bfloat16 or float16 (gather in FP32)."""
+    cas_volume may be float32, bfloat16 or float16 (gather in FP32).  check:
+    None checks the volume and table on this call (check_volume) and uses the
+    flag; a flag from check_volume for this volume and table is used as it is;
+    False gathers every tap of every stencil.  The results are the same."""
     if cas_volume.dtype not in _DTYPES:
         raise ValueError(f"unsupported CAS volume dtype {cas_volume.dtype}")
     v = _cube_edge(cas_volume, "cas_volume")
@@ -115,8 +143,18 @@ def forward_project(cas_volume: torch.Tensor, axes: torch.Tensor, m: int, mask_r
     ker = kb_table(cas_volume.device) if ker is None else ker
     _check(ker, "ker")
     mr = default_mask_radius(m) if mask_radius is None else float(mask_radius)
-    _lib.call("mgg_op_forward_project", _ptr(cas_volume), _DTYPES[cas_volume.dtype], v, _ptr(axes), p,
-              _ptr(out), m, mr, _ptr(ker), ker.numel(), float(hwidth), _stream())
+    if check is False:
+        _lib.call("mgg_op_forward_project", _ptr(cas_volume), _DTYPES[cas_volume.dtype], v, _ptr(axes), p,
+                  _ptr(out), m, mr, _ptr(ker), ker.numel(), float(hwidth), _stream())
+        return out
+    if check is None:
+        check = check_volume(cas_volume, ker)
+    else:
+        _need_numel(check, 1, "check")
+        _same_device(("cas_volume", cas_volume), ("check", check))
+        _check(check, "check", torch.int32)
+    _lib.call("mgg_op_forward_project_checked", _ptr(cas_volume), _DTYPES[cas_volume.dtype], v, _ptr(axes), p,
+              _ptr(out), m, mr, _ptr(ker), ker.numel(), float(hwidth), _ptr(check), _stream())
     return out
 
 

@@ -1,7 +1,9 @@
 """Device-resident projection pipeline over a fixed direction set.
 
 One step is the plain drop-in workload with every input resident in HBM:
-forward projection of all directions through to images
+a check of the CAS volume and table (ops.check_volume, which lets the forward
+gather skip the stencil taps whose weight is 0), forward projection of all
+directions through to images
 (``MultiGPUGridder_Matlab_Class.m:198-243``: gather, then CASImgsToImgs), then
 back projection of those images into a zeroed CAS volume (``:245-270``:
 ImgsToCASImgs, then the gather's reference-semantics transpose).  The
@@ -40,6 +42,7 @@ class ProjectionPipeline:
         self.cas_bufs = [torch.empty((self.chunk, self.m, self.m), device=self.dev) for _ in self.streams]
         self.imgs = torch.empty((self.p, self.n, self.n), device=self.dev)
         self.bp_vol = torch.zeros((self.v, self.v, self.v), device=self.dev)
+        self.check = torch.empty(1, device=self.dev, dtype=torch.int32)
         self.graph = None
 
     def chunks(self):
@@ -51,6 +54,9 @@ class ProjectionPipeline:
         group ("fp", "fp_chain", "bp_chain", "bp"), e.g. with HIP events."""
         run = timed if timed is not None else (lambda name, fn, st: fn())
         main = torch.cuda.current_stream(self.dev)
+        # the volume and table check behind the forward gather's shortcut,
+        # once per step (the volume may have changed since the last one)
+        ops.check_volume(self.cas_vol, self.ker, out=self.check)
         go = torch.cuda.Event()
         go.record(main)
         for st in self.streams:
@@ -64,7 +70,8 @@ class ProjectionPipeline:
             with torch.cuda.stream(st):
                 if prev is not None:
                     st.wait_event(prev)  # projector launches do not overlap each other
-                run("fp", lambda: ops.forward_project(self.cas_vol, ax, m, ker=self.ker, out=buf[:c]), st)
+                run("fp", lambda: ops.forward_project(self.cas_vol, ax, m, ker=self.ker, out=buf[:c],
+                                                      check=self.check), st)
                 prev = torch.cuda.Event()
                 prev.record(st)
                 run("fp_chain", lambda: ops.cas_images_to_images(buf[:c], n, out=self.imgs[s0:s0 + c],

@@ -80,6 +80,64 @@ def test_forward_project_non_finite_voxels(cuda, oracle, ker32):
     assert rel_l2(got[fin], ref[fin]) < TOL
 
 
+def test_check_volume_flag(cuda, oracle, ker32):
+    """ops.check_volume writes 1 only for a finite volume with the 501-entry
+    table whose end entries are 0; each way of breaking that gives 0."""
+    import torch
+
+    from multigpugridder_amd import ops
+
+    vol, cas, axes, m, v = _setup(16, 4, 3, oracle)
+    k = _t(ker32, cuda)
+
+    def flag(c, kk=k):
+        return int(ops.check_volume(c, kk).item())
+
+    base = _t(cas, cuda)
+    assert flag(base) == 1
+    for bad in (np.nan, np.inf, -np.inf):
+        for at in ((0, 0, 0), (v - 1, v - 1, v - 1), (v // 2, 3, v - 2)):
+            c2 = cas.copy()
+            c2[at] = bad
+            assert flag(_t(c2, cuda)) == 0, (bad, at)
+    for dt in (torch.bfloat16, torch.float16):
+        d = base.to(dt)
+        assert flag(d) == 1
+        for bad in (float("inf"), float("nan")):
+            d2 = d.clone()
+            d2[3, 4, 5] = bad
+            assert flag(d2) == 0, (dt, bad)
+    for idx, val in ((0, 1e-7), (-1, 1e-7), (250, np.inf), (17, np.nan)):
+        kb = ker32.copy()
+        kb[idx] = val
+        assert flag(base, _t(kb, cuda)) == 0, (idx, val)
+    assert flag(base, _t(oracle.kb_table(2, 301).astype(np.float32), cuda)) == 0
+    # a volume that does not start on a 16-byte boundary (the scalar loop)
+    flat = torch.zeros(v ** 3 + 1, device=cuda)
+    flat[1:] = base.reshape(-1)
+    assert flag(flat[1:].view(v, v, v)) == 1
+    flat[1 + v ** 3 // 2] = float("nan")
+    assert flag(flat[1:].view(v, v, v)) == 0
+
+
+@pytest.mark.parametrize("n,p,grid", [(128, 16, False), (128, 0, (5, 6)), (64, 48, False), (30, 12, False)])
+def test_forward_project_window_shortcut(cuda, oracle, ker32, n, p, grid):
+    """With the check flag set the gather reads 4x4x4 windows from smaller
+    boxes: it matches the full-stencil gather (only the order in which split
+    tiles add their two halves can differ) and the oracle."""
+    from multigpugridder_amd import ops
+
+    print(f"FP window shortcut n={n} p={p} grid={grid}", flush=True)
+    vol, cas, axes, m, v = _setup(n, p, 7, oracle, grid)
+    c, a, k = _t(cas, cuda), _t(axes, cuda), _t(ker32, cuda)
+    flag = ops.check_volume(c, k)
+    assert int(flag.item()) == 1
+    short = _np(ops.forward_project(c, a, m, ker=k, check=flag))
+    full = _np(ops.forward_project(c, a, m, ker=k, check=False))
+    assert_parity(short, full, tol=1e-6, peak=1e-6)
+    assert_parity(short, oracle.forward_project(cas, axes, ker32, m))
+
+
 def test_forward_project_golden(cuda):
     import os
 
