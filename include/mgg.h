@@ -187,6 +187,20 @@ int mgg_op_back_project(float *cas_volume, float *density, int v, const float *a
 int mgg_op_back_project_adjoint(float *cas_volume, int v, const float *axes, int n_axes,
                                 const float *cas_images, int m, float mask_radius,
                                 const float *ker, int ker_size, float ker_hwidth, void *stream);
+/* mgg_op_check_volume for n_images CAS images of m x m (FP32): *check = 1
+ * when ker holds 501 finite entries with zero ends and every image value is
+ * finite, else 0 (cas_images may be NULL when n_images is 0).  No reference
+ * counterpart. */
+int mgg_op_check_images(const float *cas_images, int n_images, int m, const float *ker, int ker_size,
+                        unsigned *check, void *stream);
+/* mgg_op_back_project_adjoint with mgg_op_check_images' flag for these
+ * cas_images and ker (NULL = no flag): where it reads 1 the scatter skips each
+ * stencil axis' end tap of weight 0, as mgg_op_forward_project_checked does
+ * (same sums, smaller boxes).  No reference counterpart. */
+int mgg_op_back_project_adjoint_checked(float *cas_volume, int v, const float *axes, int n_axes,
+                                        const float *cas_images, int m, float mask_radius,
+                                        const float *ker, int ker_size, float ker_hwidth,
+                                        const unsigned *check, void *stream);
 /* Gridders/gpuGridder.cpp:432-518: n CAS images (m x m) -> images (nn x nn).
  * Uses a batched rocFFT C2R; cas_images is clobbered (used as scratch). */
 int mgg_op_cas_images_to_images(float *cas_images, float *images, int n, int m, int nn,

@@ -95,6 +95,8 @@ _SIGS = {
     "mgg_op_check_volume": [_vp, _ci, _ci, _vp, _ci, _vp, _vp],
     "mgg_op_back_project": [_vp, _vp, _ci, _vp, _ci, _vp, _ci, _cf, _vp, _ci, _cf, _vp],
     "mgg_op_back_project_adjoint": [_vp, _ci, _vp, _ci, _vp, _ci, _cf, _vp, _ci, _cf, _vp],
+    "mgg_op_back_project_adjoint_checked": [_vp, _ci, _vp, _ci, _vp, _ci, _cf, _vp, _ci, _cf, _vp, _vp],
+    "mgg_op_check_images": [_vp, _ci, _ci, _vp, _ci, _vp, _vp],
     "mgg_op_cas_images_to_images": [_vp, _vp, _ci, _ci, _ci, _vp],
     "mgg_op_cas_images_to_images_noalloc": [_vp, _vp, _ci, _ci, _ci, _vp],
     "mgg_op_images_to_cas_images": [_vp, _vp, _ci, _ci, _ci, _vp],

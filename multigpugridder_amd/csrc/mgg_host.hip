@@ -768,9 +768,9 @@ int mgg_op_back_project(float *cas_volume, float *density, int v, const float *a
     });
 }
 
-int mgg_op_back_project_adjoint(float *cas_volume, int v, const float *axes, int n_axes,
-                                const float *cas_images, int m, float mask_radius, const float *ker,
-                                int ker_size, float ker_hwidth, void *stream)
+int mgg_op_back_project_adjoint_checked(float *cas_volume, int v, const float *axes, int n_axes,
+                                        const float *cas_images, int m, float mask_radius, const float *ker,
+                                        int ker_size, float ker_hwidth, const unsigned *check, void *stream)
 {
     return guarded([&] {
         if (n_axes == 0) return;
@@ -778,8 +778,32 @@ int mgg_op_back_project_adjoint(float *cas_volume, int v, const float *axes, int
         validate_geom(v, m, ker_size, ker_hwidth);
         if (n_axes < 0) raise(MGG_ERR_INVALID_ARGUMENT, "n_axes < 0");
         ProjGeom g{v, m, mask_radius, ker, ker_size, ker_hwidth};
-        check_launch(launch_back_project_adjoint(cas_volume, g, axes, n_axes, cas_images, as_stream(stream)),
+        check_launch(launch_back_project_adjoint(cas_volume, g, axes, n_axes, cas_images, as_stream(stream),
+                                                 check),
                      "back_project_adjoint");
+    });
+}
+
+int mgg_op_back_project_adjoint(float *cas_volume, int v, const float *axes, int n_axes,
+                                const float *cas_images, int m, float mask_radius, const float *ker,
+                                int ker_size, float ker_hwidth, void *stream)
+{
+    return mgg_op_back_project_adjoint_checked(cas_volume, v, axes, n_axes, cas_images, m, mask_radius, ker,
+                                               ker_size, ker_hwidth, nullptr, stream);
+}
+
+int mgg_op_check_images(const float *cas_images, int n_images, int m, const float *ker, int ker_size,
+                        unsigned *check, void *stream)
+{
+    return guarded([&] {
+        need(ker, "ker"); need(check, "check");
+        if (n_images < 0 || m <= 0) raise(MGG_ERR_INVALID_ARGUMENT, "need n_images >= 0 and m > 0");
+        if (n_images > 0) need(cas_images, "cas_images");
+        if (ker_size <= 0 || ker_size > kMaxKerSize)
+            raise(MGG_ERR_INVALID_ARGUMENT, "kernel table size %d outside 1..%d", ker_size, kMaxKerSize);
+        check_launch(launch_check_volume(cas_images, DT_F32, (size_t)n_images * m * m, ker, ker_size, check,
+                                         as_stream(stream)),
+                     "check_images");
     });
 }
 

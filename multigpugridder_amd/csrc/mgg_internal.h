@@ -79,8 +79,12 @@ void bp_brick_order(int V, int *out);
 // True when launch_back_project takes the fast path for this geometry.
 bool bp_uses_fast_path(const ProjGeom &g);
 // Exact transpose of launch_forward_project (atomic scatter into volAcc).
+// With check pointing at launch_check_volume's flag for the nAxes * M * M CAS
+// image values and the table, the scatter skips the taps of weight 0 (same
+// sums, smaller boxes).
 hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const float *axes,
-                                       int nAxes, const float *casImgs, hipStream_t s);
+                                       int nAxes, const float *casImgs, hipStream_t s,
+                                       const unsigned *check = nullptr);
 
 // ---- image chains (2-D, batched) ----------------------------------------
 // CAS images (n x M x M) -> Hermitian half spectra (n x M x (M/2+1)) ready for

@@ -784,9 +784,13 @@ __global__ __launch_bounds__(kThreads) void fp_lds_multi_kernel(FPMultiArgs m)
 // pixel at N=128 instead of one per tap (125).  Box voxels map to the volume through the reference's
 // linear index (gpuForwardProject.cu:90), so the transpose keeps the forward
 // gather's row wrap.  A tile whose box exceeds the LDS budget runs as two 8x4
-// halves; a half that still does not fit scatters with global atomics.
+// halves; a half that still does not fit scatters with global atomics.  With
+// launch_check_volume's flag over the CAS images and the table (all finite,
+// table ends 0) it scatters only the 4x4x4 windows of the taps that can have
+// nonzero weight, from the windows' smaller box, as fp_lds_tile_body gathers
+// them: the skipped taps add w * value = 0.
 // ---------------------------------------------------------------------------
-template <bool FAST>
+template <bool FAST, bool W4>
 __device__ __forceinline__ void adj_lds_tile(const FPArgs &p, const float *__restrict__ sker,
                                              float *__restrict__ box, const float *__restrict__ img,
                                              float *__restrict__ volAcc, unsigned work, int ln)
@@ -811,6 +815,13 @@ __device__ __forceinline__ void adj_lds_tile(const FPArgs &p, const float *__res
     const float y = ref_sum2c(u1, fi, v1, fj, (float)cv);
     const float z = ref_sum2c(u2, fi, v2, fj, (float)cv);
     const int X = (int)roundf(x), Y = (int)roundf(y), Z = (int)roundf(z);
+    // Window of each axis: all five taps, or (W4, checked images and table)
+    // the four that can have nonzero weight, from floor(x) - 1 (as in
+    // fp_lds_tile_body).
+    constexpr int NW = W4 ? 4 : 5;
+    const int Xs = X - 2 + ((W4 && x >= (float)X) ? 1 : 0);
+    const int Ys = Y - 2 + ((W4 && y >= (float)Y) ? 1 : 0);
+    const int Zs = Z - 2 + ((W4 && z >= (float)Z) ? 1 : 0);
     const long long V3 = (long long)V * V * V;
     const long long VV = (long long)V * V;
     // [u v] orthonormal to 1e-4 (wave-uniform: one direction per tile)
@@ -818,32 +829,32 @@ __device__ __forceinline__ void adj_lds_tile(const FPArgs &p, const float *__res
                             fabsf(v0 * v0 + v1 * v1 + v2 * v2 - 1.f) <= 1e-4f &&
                             fabsf(u0 * v0 + u1 * v1 + u2 * v2) <= 1e-4f;
 
-    float wi[5], wj[5], wk[5];
+    float wi[NW], wj[NW], wk[NW];
 #pragma unroll
-    for (int t = 0; t < 5; t++) {
+    for (int t = 0; t < NW; t++) {
         if (FAST) {
-            wi[t] = kb_fast_fp(p.ker, (float)(X - 2 + t) - x);
-            wj[t] = kb_fast_fp(p.ker, (float)(Y - 2 + t) - y);
-            wk[t] = kb_fast_fp(p.ker, (float)(Z - 2 + t) - z);
+            wi[t] = kb_fast_fp(p.ker, (float)(Xs + t) - x);
+            wj[t] = kb_fast_fp(p.ker, (float)(Ys + t) - y);
+            wk[t] = kb_fast_fp(p.ker, (float)(Zs + t) - z);
         } else {
-            wi[t] = sker[kb_index((float)(X - 2 + t) - x, p.fW, p.kerScale, p.kerCenter, p.K)];
-            wj[t] = sker[kb_index((float)(Y - 2 + t) - y, p.fW, p.kerScale, p.kerCenter, p.K)];
-            wk[t] = sker[kb_index((float)(Z - 2 + t) - z, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wi[t] = sker[kb_index((float)(Xs + t) - x, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wj[t] = sker[kb_index((float)(Ys + t) - y, p.fW, p.kerScale, p.kerCenter, p.K)];
+            wk[t] = sker[kb_index((float)(Zs + t) - z, p.fW, p.kerScale, p.kerCenter, p.K)];
         }
     }
 
     // The reference's bounds test on the linear index, one global atomic per
-    // tap (gpuForwardProject.cu:90).
+    // tap (gpuForwardProject.cu:90); every tap of the window, as the box path.
     auto scatterGlobal = [&]() {
 #pragma unroll
-        for (int cc = 0; cc < 5; cc++)
+        for (int cc = 0; cc < NW; cc++)
 #pragma unroll
-            for (int bb = 0; bb < 5; bb++)
+            for (int bb = 0; bb < NW; bb++)
 #pragma unroll
-                for (int aa = 0; aa < 5; aa++) {
-                    const long long lin = (long long)(Z - 2 + cc) * VV + (long long)(Y - 2 + bb) * V + (X - 2 + aa);
+                for (int aa = 0; aa < NW; aa++) {
+                    const long long lin = (long long)(Zs + cc) * VV + (long long)(Ys + bb) * V + (Xs + aa);
                     const float w = wi[aa] * wj[bb] * wk[cc];
-                    if (lin < V3 && lin >= 0 && w != 0.f) unsafeAtomicAdd(volAcc + lin, w * val);
+                    if (lin < V3 && lin >= 0) unsafeAtomicAdd(volAcc + lin, w * val);
                 }
     };
 
@@ -854,10 +865,11 @@ __device__ __forceinline__ void adj_lds_tile(const FPArgs &p, const float *__res
         const float ai = (float)(i0 - ci), bi = (float)(i0 + FPT - 1 - ci);
         const float aj = (float)(j0 + rlo - ci), bj = (float)(j0 + rhi - ci);
         auto lo = [&](float uu, float vv) {
-            return (int)floorf(fminf(uu * ai, uu * bi) + fminf(vv * aj, vv * bj) + (float)cv - 1e-3f) - 2;
+            return (int)floorf(fminf(uu * ai, uu * bi) + fminf(vv * aj, vv * bj) + (float)cv - 1e-3f) - (W4 ? 1 : 2);
         };
         auto hi = [&](float uu, float vv) {
-            return (int)ceilf(fmaxf(uu * ai, uu * bi) + fmaxf(vv * aj, vv * bj) + (float)cv + 1e-3f) + 2;
+            const float e = fmaxf(uu * ai, uu * bi) + fmaxf(vv * aj, vv * bj) + (float)cv + 1e-3f;
+            return (int)(W4 ? floorf(e) : ceilf(e)) + 2;
         };
         const int bx0 = __builtin_amdgcn_readfirstlane(lo(u0, v0));
         const int by0 = __builtin_amdgcn_readfirstlane(lo(u1, v1));
@@ -865,7 +877,7 @@ __device__ __forceinline__ void adj_lds_tile(const FPArgs &p, const float *__res
         const int Dx = __builtin_amdgcn_readfirstlane(hi(u0, v0)) - bx0 + 1;
         const int Dy = __builtin_amdgcn_readfirstlane(hi(u1, v1)) - by0 + 1;
         const int Dz = __builtin_amdgcn_readfirstlane(hi(u2, v2)) - bz0 + 1;
-        if (!(Dx >= 5 && Dy >= 5 && Dz >= 5 && Dx <= 64 && Dy <= 64 && Dz <= 64)) return false;
+        if (!(Dx >= NW && Dy >= NW && Dz >= NW && Dx <= 64 && Dy <= 64 && Dz <= 64)) return false;
         const int RP = Dx, PP = Dx * Dy, total = PP * Dz;
         if (total > FPBOX) return false;
         for (int e = ln; e < total; e += 64) box[e] = 0.f;
@@ -873,25 +885,33 @@ __device__ __forceinline__ void adj_lds_tile(const FPArgs &p, const float *__res
         // Every finite in-row pixel's stencil lies inside the box (the corner
         // bound); a lane whose does not (non-finite directions) scatters
         // through the checked global path instead of outside its box.
-        const bool inBox = (unsigned)(X - 2 - bx0) <= (unsigned)(Dx - 5) &&
-                           (unsigned)(Y - 2 - by0) <= (unsigned)(Dy - 5) &&
-                           (unsigned)(Z - 2 - bz0) <= (unsigned)(Dz - 5);
+        const bool inBox = (unsigned)(Xs - bx0) <= (unsigned)(Dx - NW) &&
+                           (unsigned)(Ys - by0) <= (unsigned)(Dy - NW) &&
+                           (unsigned)(Zs - bz0) <= (unsigned)(Dz - NW);
         if (act && mine && !inBox) scatterGlobal();
-        const int key = (Z - 2 - bz0) * PP + (Y - 2 - by0) * RP + (X - 2 - bx0);
+        const int key = (Zs - bz0) * PP + (Ys - by0) * RP + (Xs - bx0);
+        // Tap value (wi * val) * (wj * wk): the forward weight times the pixel
+        // value, grouped so that only NW + NW * NW products are live across
+        // the rounds instead of one per tap (the per-tap products, hoisted
+        // out of the rounds loop, held the kernel at 225 VGPRs).
+        float wiv[NW];
+#pragma unroll
+        for (int aa = 0; aa < NW; aa++) wiv[aa] = wi[aa] * val;
         auto addStencil = [&](bool atomic) {
             float *bq = box + key;
+#pragma unroll 1
+            for (int cc = 0; cc < NW; cc++)
 #pragma unroll
-            for (int cc = 0; cc < 5; cc++)
-#pragma unroll
-                for (int bb = 0; bb < 5; bb++) {
+                for (int bb = 0; bb < NW; bb++) {
                     float *rp = bq + cc * PP + bb * RP;
+                    const float wjk = wj[bb] * wk[cc];
 #pragma unroll
-                    for (int aa = 0; aa < 5; aa++) {
-                        const float w = wi[aa] * wj[bb] * wk[cc];  // the forward gather's weight
+                    for (int aa = 0; aa < NW; aa++) {
+                        const float t = wiv[aa] * wjk;
                         if (atomic) {
-                            atomicAdd(rp + aa, w * val);
+                            atomicAdd(rp + aa, t);
                         } else {
-                            rp[aa] = rp[aa] + w * val;
+                            rp[aa] = rp[aa] + t;
                             // Keep this tap's write before the next tap's read:
                             // another lane's stencil may cover this word at a
                             // later tap, and its read must see this write.
@@ -909,12 +929,13 @@ __device__ __forceinline__ void adj_lds_tile(const FPArgs &p, const float *__res
             // barrier), so a lane's read sees every earlier write of every
             // lane; only lanes of the same instruction can lose an update,
             // and those target the same word at every tap exactly when their
-            // stencil centres (X, Y, Z) coincide.  For an orthonormal frame two pixels whose rounded
-            // positions coincide differ by less than 1 in every coordinate,
-            // so they are at most sqrt(3) apart: tile neighbours.  A lane waits
-            // while a lower pending neighbour (left, up, up-left, up-right)
-            // has its centre; every round serves at least the lowest pending
-            // lane of each centre.
+            // windows start at the same voxel (key).  The start is X - 2 with
+            // X = roundf(x), or floor(x) - 1 with the 4-wide windows, per axis;
+            // for an orthonormal frame two pixels whose starts coincide have
+            // positions less than 1 apart in every coordinate, so they are at
+            // most sqrt(3) apart: tile neighbours.  A lane waits while a lower
+            // pending neighbour (left, up, up-left, up-right) has its key;
+            // every round serves at least the lowest pending lane of each key.
             const int ix = ln & 7, jy = ln >> 3;
             while (__ballot(pend) != 0ull) {
                 const int k = pend ? key : -1 - ln;  // distinct for lanes not pending
@@ -972,8 +993,13 @@ __global__ __launch_bounds__(kThreads) void adj_lds_kernel(FPArgs p, const float
         __syncthreads();
     }
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), ln = threadIdx.x & 63;
-    for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
-        adj_lds_tile<FAST>(p, sker, sbox[wv], img, volAcc, work, ln);
+    if (FAST && fp_win4(p.check)) {
+        for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
+            adj_lds_tile<FAST, true>(p, sker, sbox[wv], img, volAcc, work, ln);
+    } else {
+        for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
+            adj_lds_tile<FAST, false>(p, sker, sbox[wv], img, volAcc, work, ln);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -2713,7 +2739,7 @@ void bp_brick_order(int V, int *out)
 }
 
 hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const float *axes,
-                                       int nAxes, const float *casImgs, hipStream_t s)
+                                       int nAxes, const float *casImgs, hipStream_t s, const unsigned *check)
 {
     if (nAxes <= 0) return hipSuccess;
     FPArgs a;
@@ -2724,6 +2750,7 @@ hipError_t launch_back_project_adjoint(float *volAcc, const ProjGeom &g, const f
     if (a.convW == 2 && g.K <= FPKER) {
         // LDS-box scatter: 8x8 tiles, one per wave, 4 per workgroup.
         const bool fast = a.kerScale == 125.0f && a.kerCenter == 250.0f && g.K == 501;
+        a.check = fast ? check : nullptr;
         a.tilesPerRow = (g.M + FPT - 1) / FPT;
         a.tilesPerImg = a.tilesPerRow * a.tilesPerRow;
         a.maskD2 = mask_d2(a.maskR, g.V);
@@ -2951,6 +2978,7 @@ hipError_t launch_check_volume(const void *vol, int dtype, size_t n, const float
                                hipStream_t s)
 {
     hipLaunchKernelGGL(k_check_table, dim3(1), dim3(kThreads), 0, s, ker, K, check);
+    if (n == 0) return hipGetLastError();
     const dim3 grid(std::min(grid_for(n), 4096u)), block(kThreads);
     switch (dtype) {
     case DT_F32: hipLaunchKernelGGL(k_check_volume<DT_F32>, grid, block, 0, s, vol, n, check); break;

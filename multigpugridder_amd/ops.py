@@ -188,10 +188,34 @@ def back_project(cas_images: torch.Tensor, axes: torch.Tensor, v: int, mask_radi
     return out
 
 
+def check_images(cas_images: torch.Tensor, ker: torch.Tensor | None = None,
+                 out: torch.Tensor | None = None) -> torch.Tensor:
+    """check_volume for a (P, M, M) stack of FP32 CAS images: one-element int32
+    device flag, 1 when ker is the 501-entry table with finite entries and
+    zero ends and every image value is finite."""
+    pi, m = _stack_edge(cas_images, "cas_images")
+    if out is not None:
+        _need_numel(out, 1, "out")
+    _same_device(("cas_images", cas_images), ("ker", ker), ("out", out))
+    _check(cas_images, "cas_images")
+    ker = kb_table(cas_images.device) if ker is None else ker
+    _check(ker, "ker")
+    if out is None:
+        out = torch.empty(1, device=cas_images.device, dtype=torch.int32)
+    _check(out, "out", torch.int32)
+    _lib.call("mgg_op_check_images", _ptr(cas_images), pi, m, _ptr(ker), ker.numel(), _ptr(out), _stream())
+    return out
+
+
 def back_project_adjoint(cas_images: torch.Tensor, axes: torch.Tensor, v: int, mask_radius=None,
                          ker: torch.Tensor | None = None, hwidth: float = 2.0,
-                         out: torch.Tensor | None = None) -> torch.Tensor:
-    """Exact transpose of :func:`forward_project`, accumulated into ``out``."""
+                         out: torch.Tensor | None = None, check=None) -> torch.Tensor:
+    """Exact transpose of :func:`forward_project`, accumulated into ``out``.
+
+    check: None checks the images and table on this call (check_images) and
+    lets the scatter skip the taps of weight 0; a flag from check_images for
+    these images and table is used as it is; False scatters every tap.  The
+    results are the same."""
     pi, m = _stack_edge(cas_images, "cas_images")
     p = _n_axes(axes)
     if pi < p:
@@ -209,8 +233,20 @@ def back_project_adjoint(cas_images: torch.Tensor, axes: torch.Tensor, v: int, m
     ker = kb_table(cas_images.device) if ker is None else ker
     _check(ker, "ker")
     mr = default_mask_radius(m) if mask_radius is None else float(mask_radius)
-    _lib.call("mgg_op_back_project_adjoint", _ptr(out), v, _ptr(axes), p, _ptr(cas_images), m, mr,
-              _ptr(ker), ker.numel(), float(hwidth), _stream())
+    if p == 0:
+        return out
+    if check is False:
+        _lib.call("mgg_op_back_project_adjoint", _ptr(out), v, _ptr(axes), p, _ptr(cas_images), m, mr,
+                  _ptr(ker), ker.numel(), float(hwidth), _stream())
+        return out
+    if check is None:
+        check = check_images(cas_images[:p], ker)
+    else:
+        _need_numel(check, 1, "check")
+        _same_device(("cas_images", cas_images), ("check", check))
+        _check(check, "check", torch.int32)
+    _lib.call("mgg_op_back_project_adjoint_checked", _ptr(out), v, _ptr(axes), p, _ptr(cas_images), m, mr,
+              _ptr(ker), ker.numel(), float(hwidth), _ptr(check), _stream())
     return out
 
 

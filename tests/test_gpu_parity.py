@@ -335,6 +335,35 @@ def test_back_project_adjoint(cuda, oracle, ker32, n, p, scale):
     assert_parity(_np(out), oracle.back_project_adjoint(y, axes, ker32, v))
 
 
+@pytest.mark.parametrize("n,p,scale", [(64, 24, 1.0), (32, 20, 1.3)])
+def test_back_project_adjoint_window_shortcut(cuda, oracle, ker32, n, p, scale):
+    """With the image check flag set the transpose scatters 4x4x4 windows:
+    the same volume as the full-stencil scatter (up to the order of the
+    global atomic adds), and a non-finite image value clears the flag."""
+    import torch
+
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.synth import random_directions
+
+    print(f"adjoint window shortcut n={n} p={p} scale={scale}", flush=True)
+    m, v = 2 * n, 2 * n + 6
+    axes = random_directions(p, 19)
+    axes[1::2] *= np.float32(scale)
+    y = np.random.default_rng(4).standard_normal((p, m, m)).astype(np.float32)
+    yt, at, k = _t(y, cuda), _t(axes, cuda), _t(ker32, cuda)
+    flag = ops.check_images(yt, k)
+    assert int(flag.item()) == 1
+    short = torch.zeros((v, v, v), device=cuda)
+    full = torch.zeros((v, v, v), device=cuda)
+    ops.back_project_adjoint(yt, at, v, ker=k, out=short, check=flag)
+    ops.back_project_adjoint(yt, at, v, ker=k, out=full, check=False)
+    assert_parity(_np(short), _np(full), tol=1e-6, peak=1e-6)
+    assert_parity(_np(short), oracle.back_project_adjoint(y, axes, ker32, v))
+    y2 = y.copy()
+    y2[p // 2, m // 2, m // 2 + 3] = np.inf
+    assert int(ops.check_images(_t(y2, cuda), k).item()) == 0
+
+
 def test_kernels_follow_the_contracted_reading(cuda, oracle, ker32):
     """The kernels take their rounding decisions as the reference compiled by
     nvcc with its default --fmad=true does (the reading the oracle pins,

@@ -103,6 +103,10 @@ def test_device_ops_validate_before_launch():
     assert rc == _lib.MGG_ERR_INVALID_ARGUMENT
     rc = lib.mgg_op_check_volume(None, 0, 38, None, 501, None, None)
     assert rc == _lib.MGG_ERR_INVALID_ARGUMENT
+    rc = lib.mgg_op_check_images(None, 4, 32, None, 501, None, None)
+    assert rc == _lib.MGG_ERR_INVALID_ARGUMENT
+    rc = lib.mgg_op_back_project_adjoint_checked(None, 38, None, 4, None, 32, 15.0, None, 501, 2.0, None, None)
+    assert rc == _lib.MGG_ERR_INVALID_ARGUMENT
     rc = lib.mgg_op_cas_images_to_images(None, None, 4, 33, 16, None)
     assert rc == _lib.MGG_ERR_INVALID_ARGUMENT
     rc = lib.mgg_filter_divide_scalar(None, ctypes.c_float(0.0), 10, None)
