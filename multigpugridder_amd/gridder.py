@@ -21,6 +21,12 @@ Differences from the Matlab class (all deliberate):
   back-projection pass.
 * ``interpFactor`` is kept as given; the reference silently rescales it for
   non-power-of-two N (.m:69-83).  Pass ``pow2_interp=True`` for that behaviour.
+* The class's own arrays (Volume, Images, CASImages, CASVolume) live in pinned
+  host memory, so the device copies run at full PCIe rate; the reference pins
+  the Matlab arrays in place instead (cudaHostRegister, AbstractGridder.cpp:
+  121-157), which this platform does not offer for existing memory.  Falls
+  back to ordinary numpy arrays when pinned memory is not available
+  (``pinned=False`` asks for them).
 """
 from __future__ import annotations
 
@@ -30,6 +36,19 @@ import numpy as np
 
 from . import _lib
 from .hostmath import cas_imgs_from_imgs, imgs_from_cas_imgs, kb_table
+from .server import pinned_empty
+
+
+def _host_zeros(shape, pinned: bool) -> np.ndarray:
+    """Zeroed float32 host array, pinned when asked and available."""
+    if pinned:
+        try:
+            a = pinned_empty(shape)
+            a[...] = 0.0
+            return a
+        except _lib.MggError:
+            pass
+    return np.zeros(shape, dtype=np.float32)
 
 
 class MultiGPUGridder:
@@ -38,7 +57,7 @@ class MultiGPUGridder:
     extraPadding = 3
 
     def __init__(self, VolumeSize, NumAxes, interpFactor=2, GPUs=None, RunFFTOnGPU=1, verbose=0,
-                 nStreamsFP=10, nStreamsBP=4, MaxAxesToAllocate=1000, pow2_interp=False):
+                 nStreamsFP=10, nStreamsBP=4, MaxAxesToAllocate=1000, pow2_interp=False, pinned=True):
         if not VolumeSize:
             raise ValueError("VolumeSize is a required input.")
         if not NumAxes:
@@ -66,13 +85,13 @@ class MultiGPUGridder:
         self.V = self.M + 2 * self.extraPadding
         self.MaskRadius = np.float32((np.float32(n) * self.interpFactor) / 2 - 1)
 
-        self.Volume = np.zeros((n, n, n), dtype=np.float32)
-        self.Images = np.zeros((self.NumAxes, n, n), dtype=np.float32)
+        self.Volume = _host_zeros((n, n, n), pinned)
+        self.Images = _host_zeros((self.NumAxes, n, n), pinned)
         self.KBTable = kb_table(self.kerHWidth, self.kerTblSize)
         self.CASImages = None
         if not self.RunFFTOnGPU or self.verbose:
-            self.CASImages = np.zeros((self.NumAxes, self.M, self.M), dtype=np.float32)
-        self.CASVolume = np.zeros((self.V, self.V, self.V), dtype=np.float32)
+            self.CASImages = _host_zeros((self.NumAxes, self.M, self.M), pinned)
+        self.CASVolume = _host_zeros((self.V, self.V, self.V), pinned)
         self.coordAxes = np.zeros((self.NumAxes, 9), dtype=np.float32)
         self._numAxes = self.NumAxes
 
