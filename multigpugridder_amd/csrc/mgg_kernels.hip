@@ -1407,20 +1407,24 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                     for (int jj = 0; jj < 5; jj++) {
                         const float dj = (float)(J - 2 + jj - ci);
                         const float Bx = dj * v0, By = dj * v1, Bz = dj * v2;
-                        float val[5];
+                        float val[5], vin[5];  // image values; 1 where the tap is in bounds, else 0
                         const int row = (J - 2 + jj) * M + (I - 2);
                         if (inside) {
                             f4 a;
                             __builtin_memcpy(&a, im + row, 16);
                             val[0] = a.x; val[1] = a.y; val[2] = a.z; val[3] = a.w;
                             val[4] = im[row + 4];
+#pragma unroll
+                            for (int ii = 0; ii < 5; ii++) vin[ii] = 1.f;
                         } else {
                             // Reference bound on the linear index only
                             // (gpuBackProject.cu:99): wraps across rows.
 #pragma unroll
                             for (int ii = 0; ii < 5; ii++) {
                                 const int lin = row + ii;
-                                val[ii] = (lin >= 0 && lin < M2) ? im[lin] : 0.f;
+                                const bool ok = lin >= 0 && lin < M2;
+                                val[ii] = ok ? im[lin] : 0.f;
+                                vin[ii] = ok ? 1.f : 0.f;
                             }
                         }
 #pragma unroll
@@ -1447,10 +1451,9 @@ __global__ __launch_bounds__(kThreads) void bp_fast_kernel(BPArgs p)
                             w = w * wy;
                             w = w * wz;
                             cum = fmaf(val[ii], w, cum);
-                            if (DENS) {
-                                const int lin = row + ii;
-                                dsum += (inside || (lin >= 0 && lin < M2)) ? w : 0.f;
-                            }
+                            // the density: the back projection of all-ones images
+                            // (fma(1, w, d) = d + w, fma(0, w, d) = d)
+                            if (DENS) dsum = fmaf(vin[ii], w, dsum);
                         }
                     }
                     // Two queued entries of one group can name the same voxel
