@@ -176,6 +176,42 @@ def test_legacy_ctypes_symbols(cuda, oracle, ker32):
     lib.mgg_destroy(h)
 
 
+def test_object_forward_keeps_non_finite_semantics(cuda, oracle, ker32):
+    """The gridder object checks its CAS volume on every ForwardProject: a
+    NaN and an Inf voxel switch its gather to the full stencils, so they reach
+    exactly the pixels whose stencil holds them, as in the reference (the
+    legacy flow: CAS volume in, CAS images out)."""
+    from multigpugridder_amd import _lib
+    from multigpugridder_amd.synth import random_directions
+
+    lib = _lib.lib()
+    v = m = 32
+    cas = oracle.vol_to_cas(np.random.default_rng(1).random((10, 10, 10)), 2.6, 3)[:v, :v, :v]
+    cas = np.ascontiguousarray(cas, dtype=np.float32)
+    c = v // 2
+    cas[c + 2, c - 1, c] = np.nan
+    cas[c - 3, c + 4, c + 1] = np.inf
+    axes = random_directions(12, 5)
+    h = lib.Gridder_new()
+    assert h
+    fp = ctypes.POINTER(ctypes.c_float)
+    assert lib.SetNumberGPUs(h, 1) and lib.SetNumberStreams(h, 3)
+    assert lib.SetAxes(h, axes.ctypes.data_as(fp), _lib.int_array([axes.size, 1, 1]))
+    assert lib.SetVolume(h, cas.ctypes.data_as(fp), _lib.int_array([v, v, v]))
+    assert lib.SetImgSize(h, _lib.int_array([m, m, m]))
+    r = ctypes.c_float(m / 2 - 2)
+    assert lib.SetMaskRadius(h, ctypes.byref(r))
+    assert lib.Forward_Project(h)
+    out = np.ctypeslib.as_array(lib.GetImages(h), shape=(12, m, m)).copy()
+    lib.mgg_destroy(h)
+    ref = oracle.forward_project(cas, axes, ker32, m, mask_radius=m / 2 - 2)
+    assert np.isnan(ref).any() and np.isinf(ref).any()
+    assert np.array_equal(np.isnan(out), np.isnan(ref))
+    assert np.array_equal(np.isinf(out), np.isinf(ref))
+    fin = np.isfinite(ref)
+    assert rel_l2(out[fin], ref[fin]) < TOL
+
+
 def test_config_c1_n64_500_random_directions(cuda, oracle, ker32):
     """The CPU-runnable configuration end to end through the gridder object:
     blob volume 64^3 (seed 1), interp 2, 500 random-quaternion directions
