@@ -147,7 +147,7 @@ def forward_project(cas_volume: torch.Tensor, axes: torch.Tensor, m: int, mask_r
         _lib.call("mgg_op_forward_project", _ptr(cas_volume), _DTYPES[cas_volume.dtype], v, _ptr(axes), p,
                   _ptr(out), m, mr, _ptr(ker), ker.numel(), float(hwidth), _stream())
         return out
-    if check is None:
+    if check is None or check is True:
         check = check_volume(cas_volume, ker)
     else:
         _need_numel(check, 1, "check")
@@ -239,7 +239,7 @@ def back_project_adjoint(cas_images: torch.Tensor, axes: torch.Tensor, v: int, m
         _lib.call("mgg_op_back_project_adjoint", _ptr(out), v, _ptr(axes), p, _ptr(cas_images), m, mr,
                   _ptr(ker), ker.numel(), float(hwidth), _stream())
         return out
-    if check is None:
+    if check is None or check is True:
         check = check_images(cas_images[:p], ker)
     else:
         _need_numel(check, 1, "check")
