@@ -747,8 +747,8 @@ __global__ __launch_bounds__(kThreads) void adj_kernel(FPArgs p, const float *__
 // finds its segment by a binary search over the segments' first work items
 // (wave-uniform, so the search and the descriptor reads are scalar).  The
 // tile code is fp_lds_tile's, so each image is bit-identical to a launch of
-// its own volume alone.  The launch is persistent (no more workgroups than are
-// resident at once), so a batch of small requests is one long-running launch.
+// its own volume alone.  A batch of requests on several volumes is one launch;
+// its workgroups walk the work list three tiles per wave.
 // ---------------------------------------------------------------------------
 template <int DT>
 __global__ __launch_bounds__(kThreads) void fp_lds_multi_kernel(FPMultiArgs m)
@@ -2694,15 +2694,10 @@ hipError_t launch_forward_project_multi(const FPMultiArgs &m, hipStream_t s)
 {
     if (m.nSegs <= 0 || m.totalWork == 0) return hipSuccess;
     if (m.nSegs > kFpMaxSegments) return hipErrorInvalidValue;
-    // Persistent: at most as many workgroups as the GPU holds at once (four per
-    // CU: 40 KB of LDS and four waves per SIMD each); every workgroup walks the
-    // work list until it is exhausted.
-    int dev = 0, cus = 0;
-    hipError_t e = hipGetDevice(&dev);
-    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    if (e != hipSuccess) return e;
-    const unsigned resident = 4u * (unsigned)std::max(cus, 1);
-    const unsigned grid = std::min((m.totalWork + 3) / 4, resident);
+    // The single-volume launch's grid: three tiles per wave, a grid-stride
+    // apart.  A persistent grid (no more workgroups than are resident, each
+    // walking the list) ran 25% slower on the same tiles.
+    const unsigned grid = (m.totalWork + 11) / 12;
     hipLaunchKernelGGL(fp_lds_multi_kernel<DT_F32>, dim3(grid), dim3(kThreads), 0, s, m);
     return hipGetLastError();
 }
