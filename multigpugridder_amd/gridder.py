@@ -39,6 +39,21 @@ from .hostmath import cas_imgs_from_imgs, imgs_from_cas_imgs, kb_table
 from .server import pinned_empty
 
 
+def _copy_into(dst: np.ndarray, src) -> None:
+    """``dst[...] = src`` over the host's threads.  An image stack is 655 MB at
+    10,000 directions and N=128; numpy copies it on one thread (50 ms on an
+    MI355X host, a third of forwardProject), torch's CPU copy on all of them
+    (13 ms).  Other shapes, dtypes and layouts go through numpy."""
+    src = np.asarray(src)
+    t = _lib.torch
+    if (t is not None and src.shape == dst.shape and src.dtype == dst.dtype == np.float32
+            and src.flags.c_contiguous and src.flags.writeable and dst.flags.c_contiguous
+            and dst.flags.writeable and dst.size >= (1 << 20)):
+        t.from_numpy(dst).copy_(t.from_numpy(src))
+    else:
+        np.copyto(dst, src, casting="same_kind")
+
+
 def _host_zeros(shape, pinned: bool) -> np.ndarray:
     """Zeroed float32 host array, pinned when asked and available."""
     if pinned:
@@ -178,7 +193,9 @@ class MultiGPUGridder:
         p = self._numAxes
         if not self.RunFFTOnGPU:
             self.Images[:p] = imgs_from_cas_imgs(self.CASImages[:p], self.VolumeSize)
-        return self.Images[:p].copy()
+        out = np.empty((p,) + self.Images.shape[1:], dtype=np.float32)
+        _copy_into(out, self.Images[:p])
+        return out
 
     # -- backProject (.m:245-270) ---------------------------------------------
     def backProject(self, images=None, axes=None):
@@ -186,7 +203,7 @@ class MultiGPUGridder:
             images = np.asarray(images, dtype=np.float32)
             if images.ndim == 2:
                 images = images[None]
-            self.Images[:images.shape[0]] = images
+            _copy_into(self.Images[:images.shape[0]], images)
             if axes is None:
                 raise TypeError("backProject(images, axes) needs the directions")
             self._set_axes(axes)
@@ -212,7 +229,7 @@ class MultiGPUGridder:
         """Back project, divide by the plane density (+1e-6, FP64) and return the volume."""
         if images is not None:
             images = np.asarray(images, dtype=np.float32)
-            self.Images[:images.shape[0]] = images
+            _copy_into(self.Images[:images.shape[0]], images)
             if axes is None:
                 raise TypeError("reconstructVol(images, axes) needs the directions")
             self._set_axes(axes)
