@@ -40,6 +40,30 @@ def test_version_and_error_string():
     assert isinstance(_lib.last_error(), str)
 
 
+def test_missing_library_fails_loudly():
+    """Without the built library every entry point raises: there is no CPU
+    fallback behind the gridder class, the operators or the server."""
+    import subprocess
+    import sys
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "from multigpugridder_amd import _lib\n"
+        "_lib.LIB_PATH = '/nonexistent/libMultiGPUGridder.so'\n"
+        "from multigpugridder_amd import MultiGPUGridder, ops\n"
+        "import numpy as np\n"
+        "for f in (lambda: MultiGPUGridder(16, 4, 2), lambda: _lib.call('mgg_version'),\n"
+        "          lambda: ops.kb_table(device='cpu')):\n"
+        "    try:\n"
+        "        f()\n"
+        "    except _lib.LibraryNotBuilt as e:\n"
+        "        assert 'no CPU fallback' in str(e)\n"
+        "    else:\n"
+        "        raise SystemExit('no error')\n"
+        "print('refused')\n" % ROOT)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0 and "refused" in r.stdout, r.stdout + r.stderr
+
+
 def test_kb_table_matches_oracle(oracle):
     ours = mgg.kb_table(2, 501)
     ref = oracle.kb_table(2, 501).astype(np.float32)
