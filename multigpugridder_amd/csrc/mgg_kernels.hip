@@ -21,6 +21,7 @@
 #include <hip/hip_fp16.h>
 
 #include <algorithm>
+#include <type_traits>
 
 namespace mgg {
 namespace {
@@ -338,7 +339,7 @@ __device__ __forceinline__ void load4<VolLoad<DT_F16>>(const uint16_t *p, float 
 // gives the same sums bit for bit, and the box shrinks to the voxels the
 // windows touch: [floor(min) - 1, floor(max) + 2] per axis instead of
 // [floor(min) - 2, ceil(max) + 2], about 40% fewer voxels to stage.
-template <int DT, bool FAST, bool W4>
+template <int DT, bool FAST, bool W4, bool BIG>
 __device__ __forceinline__ void fp_lds_tile_body(const FPArgs &p, const float *__restrict__ sker,
                                                  float *__restrict__ box, int a, int i0, int j0, int ln,
                                                  bool active, bool pix, float *out)
@@ -401,8 +402,11 @@ __device__ __forceinline__ void fp_lds_tile_body(const FPArgs &p, const float *_
         const int Dx = __builtin_amdgcn_readfirstlane(hi(u0, v0)) - bx0 + 1;
         const int Dy = __builtin_amdgcn_readfirstlane(hi(u1, v1)) - by0 + 1;
         const int Dz = __builtin_amdgcn_readfirstlane(hi(u2, v2)) - bz0 + 1;
+        // Volume addressing by 32-bit byte offsets: from the volume's base
+        // (below 4 GB), or (BIG) from the box's first element, whose
+        // linear-index span is below 64 V^2 elements.
         if (!(Dx >= NW && Dy >= NW && Dz >= NW && Dx <= 64 && Dy <= 64 && Dz <= 64 &&
-              V3 * (long long)sizeof(T) < (1LL << 32)))
+              (BIG ? 64LL * V * V : V3) * (long long)sizeof(T) < (1LL << 32)))
             return false;
         // 16-byte segments per box row: enough that the 8-wide read of a window
         // (below) stays inside the row's staged segments.
@@ -444,9 +448,13 @@ __device__ __forceinline__ void fp_lds_tile_body(const FPArgs &p, const float *_
         // 18.3 ms per 10,000 directions at N=128 with the 5-wide windows; 8
         // needs more than 128 VGPRs, three waves per SIMD instead of four,
         // 22.0 ms).  The checked and unchecked variants are separate loops
-        // (boxIn is wave-uniform); the unchecked one addresses the volume by a
-        // 32-bit byte offset from its base (V^3 * sizeof(T) < 2^32 is
-        // required above).
+        // (boxIn is wave-uniform).  The unchecked one addresses the volume by
+        // a 32-bit byte offset from the volume's base, or (BIG: volumes of
+        // 4 GB and more) from the box's first element; the checked one, which
+        // runs only on boxes that cross the volume's index range, by the
+        // int linear index, or (BIG) a 64-bit one.
+        const T *volBox = vol + ((BIG && boxIn) ? linLo : 0);
+        using Lin = std::conditional_t<BIG, long long, int>;
         constexpr int SB = 6;
         const int lStep = G * VV, dStep = G * PP;
         for (int c = 0; c < nLanePass; c++) {
@@ -456,23 +464,26 @@ __device__ __forceinline__ void fp_lds_tile_body(const FPArgs &p, const float *_
             const int ey = (int)(((float)pq + 0.5f) * rSegs);
             const int sg = pq - ey * segs;
             if (pl >= G) continue;
-            const int lin0 = (bz0 + pl) * VV + (by0 + ey) * V + (bx0 + 4 * sg);
+            const Lin lin0 = (Lin)(bz0 + pl) * VV + (Lin)(by0 + ey) * V + (bx0 + 4 * sg);
             float *d0 = box + pl * PP + ey * RP + 4 * sg;
             auto stage = [&](auto inBox) {
-                int l = lin0;
+                // offset from volBox (BIG and boxIn: inside the box's span,
+                // below 2^30) or the linear index
+                using Idx = std::conditional_t<decltype(inBox)::value, int, Lin>;
+                Idx l = (Idx)((BIG && decltype(inBox)::value) ? lin0 - (Lin)linLo : lin0);
                 float *dp = d0;
                 for (int ez = pl; ez < Dz; ez += SB * G) {
                     float v4[SB][4];
 #pragma unroll
                     for (int k = 0; k < SB; k++) {
                         if (ez + k * G < Dz) {
-                            const int lk = l + k * lStep;
+                            const Idx lk = l + (Idx)k * lStep;
                             if (decltype(inBox)::value) {
-                                load4<L>((const T *)((const char *)vol + (unsigned)lk * (unsigned)sizeof(T)), v4[k]);
+                                load4<L>((const T *)((const char *)volBox + (unsigned)lk * (unsigned)sizeof(T)), v4[k]);
                             } else {
 #pragma unroll
                                 for (int e = 0; e < 4; e++)
-                                    v4[k][e] = (lk + e >= 0 && lk + e < V3i) ? L::one(vol + lk + e) : 0.f;
+                                    v4[k][e] = (lk + e >= 0 && lk + e < (BIG ? (Lin)V3 : (Lin)V3i)) ? L::one(vol + lk + e) : 0.f;
                             }
                         }
                     }
@@ -636,7 +647,7 @@ __device__ __forceinline__ void fp_lds_tile_body(const FPArgs &p, const float *_
 
 // One 8x8 tile of one direction; W4 (only with the 501-entry table) selects
 // the 4-wide windows.
-template <int DT, bool FAST, bool W4>
+template <int DT, bool FAST, bool W4, bool BIG = false>
 __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__restrict__ sker, float *__restrict__ box,
                                             unsigned work, int ln)
 {
@@ -656,7 +667,7 @@ __device__ __forceinline__ void fp_lds_tile(const FPArgs &p, const float *__rest
         if (pix) *out = 0.f;
         return;
     }
-    fp_lds_tile_body<DT, FAST, FAST && W4>(p, sker, box, a, i0, j0, ln, active, pix, out);
+    fp_lds_tile_body<DT, FAST, FAST && W4, BIG>(p, sker, box, a, i0, j0, ln, active, pix, out);
 }
 
 // The 4-wide windows need a finite volume and a finite 501-entry table whose
@@ -667,7 +678,7 @@ __device__ __forceinline__ bool fp_win4(const unsigned *check)
     return check != nullptr && __builtin_amdgcn_readfirstlane(*check) == 1u;
 }
 
-template <int DT, bool FAST>
+template <int DT, bool FAST, bool BIG>
 __global__ __launch_bounds__(kThreads) void fp_lds_kernel(FPArgs p)
 {
     __shared__ float sker[FPKER];
@@ -684,10 +695,10 @@ __global__ __launch_bounds__(kThreads) void fp_lds_kernel(FPArgs p)
     // loop is register-allocated on its own.
     if (FAST && fp_win4(p.check)) {
         for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
-            fp_lds_tile<DT, FAST, true>(p, sker, sbox[wv], work, ln);
+            fp_lds_tile<DT, FAST, true, BIG>(p, sker, sbox[wv], work, ln);
     } else {
         for (unsigned work = blockIdx.x * 4u + (unsigned)wv; work < p.totalWork; work += gridDim.x * 4u)
-            fp_lds_tile<DT, FAST, false>(p, sker, sbox[wv], work, ln);
+            fp_lds_tile<DT, FAST, false, BIG>(p, sker, sbox[wv], work, ln);
     }
 }
 
@@ -750,7 +761,7 @@ __global__ __launch_bounds__(kThreads) void adj_kernel(FPArgs p, const float *__
 // its own volume alone.  A batch of requests on several volumes is one launch;
 // its workgroups walk the work list three tiles per wave.
 // ---------------------------------------------------------------------------
-template <int DT>
+template <int DT, bool BIG>
 __global__ __launch_bounds__(kThreads) void fp_lds_multi_kernel(FPMultiArgs m)
 {
     __shared__ __attribute__((aligned(16))) float sbox[4][FPBOX];
@@ -768,8 +779,8 @@ __global__ __launch_bounds__(kThreads) void fp_lds_multi_kernel(FPMultiArgs m)
         a.maskR = 0.f; a.kerScale = 125.0f; a.kerCenter = 250.0f; a.fW = 2.0f;
         a.convW = 2; a.tilesPerRow = sg.tilesPerRow; a.tilesPerImg = sg.tilesPerImg;
         a.perXcd = 0; a.totalWork = 0; a.maskD2 = sg.maskD2; a.check = sg.check;
-        if (fp_win4(sg.check)) fp_lds_tile<DT, true, true>(a, nullptr, sbox[wv], work - sg.workStart, ln);
-        else fp_lds_tile<DT, true, false>(a, nullptr, sbox[wv], work - sg.workStart, ln);
+        if (fp_win4(sg.check)) fp_lds_tile<DT, true, true, BIG>(a, nullptr, sbox[wv], work - sg.workStart, ln);
+        else fp_lds_tile<DT, true, false, BIG>(a, nullptr, sbox[wv], work - sg.workStart, ln);
     }
 }
 
@@ -2632,9 +2643,15 @@ hipError_t launch_forward_project(const void *vol, int dtype, const ProjGeom &g,
         // one tile per wave, 11.4 / 11.9 / 11.3 / 11.6 ms with 2 / 4 / 6 / 8
         // tiles per wave, 13.5 ms with a persistent 1,024-workgroup grid.
         const dim3 grid((a.totalWork + 11) / 12), block(kThreads);
+        // volumes of 4 GB and more: box addressing from the box's first element
+        const int esz = dtype == DT_F32 ? 4 : 2;
+        const bool big = (long long)g.V * g.V * g.V * esz >= (1LL << 32);
 #define MGG_FP_LAUNCH(DTV)                                                                   \
-    if (fast) hipLaunchKernelGGL((fp_lds_kernel<DTV, true>), grid, block, 0, s, a);          \
-    else hipLaunchKernelGGL((fp_lds_kernel<DTV, false>), grid, block, 0, s, a);
+    if (big) {                                                                               \
+        if (fast) hipLaunchKernelGGL((fp_lds_kernel<DTV, true, true>), grid, block, 0, s, a);  \
+        else hipLaunchKernelGGL((fp_lds_kernel<DTV, false, true>), grid, block, 0, s, a);      \
+    } else if (fast) hipLaunchKernelGGL((fp_lds_kernel<DTV, true, false>), grid, block, 0, s, a); \
+    else hipLaunchKernelGGL((fp_lds_kernel<DTV, false, false>), grid, block, 0, s, a);
         switch (dtype) {
         case DT_F32: MGG_FP_LAUNCH(DT_F32) break;
         case DT_BF16: MGG_FP_LAUNCH(DT_BF16) break;
@@ -2698,7 +2715,10 @@ hipError_t launch_forward_project_multi(const FPMultiArgs &m, hipStream_t s)
     // apart.  A persistent grid (no more workgroups than are resident, each
     // walking the list) ran 25% slower on the same tiles.
     const unsigned grid = (m.totalWork + 11) / 12;
-    hipLaunchKernelGGL(fp_lds_multi_kernel<DT_F32>, dim3(grid), dim3(kThreads), 0, s, m);
+    bool big = false;  // a volume of 4 GB or more
+    for (int i = 0; i < m.nSegs; i++) big |= (long long)m.seg[i].V * m.seg[i].V * m.seg[i].V * 4 >= (1LL << 32);
+    if (big) hipLaunchKernelGGL((fp_lds_multi_kernel<DT_F32, true>), dim3(grid), dim3(kThreads), 0, s, m);
+    else hipLaunchKernelGGL((fp_lds_multi_kernel<DT_F32, false>), dim3(grid), dim3(kThreads), 0, s, m);
     return hipGetLastError();
 }
 
