@@ -381,11 +381,13 @@ def dropin_bench(n, axes_np, reps=2):
     from multigpugridder_amd import MultiGPUGridder
     from multigpugridder_amd.synth import blob_volume
 
+    from multigpugridder_amd import _lib
+
     p = axes_np.shape[0]
     vol = blob_volume(n, 1)
     g = MultiGPUGridder(n, p, 2)
     g.setVolume(vol)
-    tf, tb = [], []
+    tf, tb, cf, cb = [], [], [], []
     for r in range(reps + 1):
         t0 = time.perf_counter()
         im = g.forwardProject(axes_np)
@@ -394,15 +396,28 @@ def dropin_bench(n, axes_np, reps=2):
         g.backProject(im, axes_np)
         g.getVol()
         t2 = time.perf_counter()
+        # the C ABI calls alone, on the class's arrays (no Python-side copies)
+        _lib.call("mgg_volume_to_cas_volume", g.objectHandle)
+        _lib.call("mgg_forward_project", g.objectHandle)
+        t3 = time.perf_counter()
+        _lib.call("mgg_reset_volume", g.objectHandle)
+        _lib.call("mgg_back_project", g.objectHandle)
+        _lib.call("mgg_get_volume", g.objectHandle)
+        t4 = time.perf_counter()
         if r:
             tf.append(t1 - t0)
             tb.append(t2 - t1)
+            cf.append(t3 - t2)
+            cb.append(t4 - t3)
     g.delete()
     f, b = float(np.median(tf)), float(np.median(tb))
     return {"fwd_dirs_per_s": round(p / f, 1), "back_dirs_per_s": round(p / b, 1),
             "fwd_back_dirs_per_s": round(p / (f + b), 1), "fwd_s": round(f, 3), "back_s": round(b, 3),
+            "c_abi_fwd_s": round(float(np.median(cf)), 4), "c_abi_back_s": round(float(np.median(cb)), 4),
             "what": f"MultiGPUGridder(N={n}).forwardProject / resetVolume + backProject + getVol on "
-                    f"{p} grid directions, host arrays (pinned staging, PCIe included), median of {reps}"}
+                    f"{p} grid directions, host arrays (pinned staging, PCIe included), median of {reps}; "
+                    "c_abi_*: mgg_volume_to_cas_volume + mgg_forward_project, and mgg_reset_volume + "
+                    "mgg_back_project + mgg_get_volume, without the class's host copies"}
 
 
 def differentiation_bench(dev, n=96, p=2048, reps=5):
