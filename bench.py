@@ -500,7 +500,7 @@ def main():
     ap.add_argument("--issue-file", default=os.path.join(ROOT, "profiles", "issue_kernels_n128.md"),
                     help="profile summary holding the projectors' VALU instructions per direction")
     ap.add_argument("--no-peak-probe", action="store_true", help="skip the VALU / LDS / copy peak probe")
-    ap.add_argument("--serve-requests", type=int, default=150,
+    ap.add_argument("--serve-requests", type=int, default=300,
                     help="requests per serving measurement (0: skip the serving leg)")
     ap.add_argument("--serve-lambdas", default="0.8",
                     help="Poisson rates as fractions of the plain path's capacity, comma separated")
