@@ -635,6 +635,22 @@ def main():
             "avg_launch_ms": round(launch_ms, 4),
         },
     }
+    # Rates beside the roofline (reported, not bounds): stencil taps per second
+    # (125 per in-mask pixel and direction, the slab model's count for both
+    # projectors: 5 x 5 x 5 per pixel forward, 25 per slab voxel back), FLOP at
+    # 4 per tap (the weight product wi*wj*wk and the multiply-add into the sum)
+    # against the FP32 vector peak, and the HBM rate of the rocprof traffic per
+    # direction at the live launch rate.
+    taps = 125 * n_mask(m)
+    rates = {"taps_per_direction": taps, "flops_per_tap": 4, "fp32_vector_peak_tflops": 157.3}
+    for name, lms in (("fp", fp_launch_ms), ("bp", bp_launch_ms)):
+        dps = chunk / (lms * 1e-3)
+        tr = measured_traffic(args.traffic_file, n, name, 1)
+        tflops = 4 * taps * dps / 1e12
+        rates[name] = {"directions_per_s": round(dps, 1), "taps_per_s": round(taps * dps, 1),
+                       "tflops": round(tflops, 2), "frac_of_fp32_peak": round(tflops / 157.3, 4),
+                       "rocprof_hbm_gbs": round(tr * dps / 1e9, 1) if tr else None}
+    out["rates"] = rates
     # Issue roofline: the projectors are gathers bound by instruction issue, so
     # their VALU rate (rocprof instruction count per direction x directions per
     # launch / live launch time) is set beside the measured VALU issue peak.
