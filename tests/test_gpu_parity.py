@@ -574,3 +574,38 @@ def test_filters_bit_exact(cuda, oracle):
     filter_call("mgg_filter_add_volume", t8, tb, a.size)
     np.testing.assert_array_equal(_np(t8), a + b)
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("n,interp,pad", [(16, 3.0, 5), (20, 2.0, 0), (24, 2.0, 1), (12, 4.0, 3)])
+def test_other_interp_factors_and_paddings(cuda, oracle, ker32, n, interp, pad):
+    """The whole path with an interpolation factor or CAS-volume padding other
+    than the defaults (the reference's kernels take any V = M + 2 pad): Vol ->
+    CAS, forward projection, CAS images -> images, images -> CAS images, back
+    projection and CAS -> Vol against the oracle.  Padding 0 puts the outer
+    stencil taps outside the volume, where only the linear-index bound applies."""
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.synth import blob_volume, random_directions
+
+    m = int(np.float32(n) * np.float32(interp))
+    v = m + 2 * pad
+    print(f"n={n} interp={interp} pad={pad}: m={m} v={v}", flush=True)
+    vol = blob_volume(n, 3)
+    axes = random_directions(9, 41)
+    dk, da = _t(ker32, cuda), _t(axes, cuda)
+    cas = ops.volume_to_cas_volume(_t(vol, cuda), interp, pad)
+    cas_ref = oracle.vol_to_cas(vol, interp, pad)
+    assert cas.shape == (v, v, v)
+    assert rel_l2(cas.cpu().numpy(), cas_ref) < 1e-5
+    cas32 = cas_ref.astype(np.float32)
+    ci = ops.forward_project(_t(cas32, cuda), da, m, ker=dk)
+    ci_ref = oracle.forward_project(cas32, axes, ker32, m)
+    assert_parity(ci.cpu().numpy(), ci_ref)
+    imgs = ops.cas_images_to_images(ci, n)
+    assert rel_l2(imgs.cpu().numpy(), oracle.cas_imgs_to_imgs(ci_ref, n)) < 1e-5
+    ci2 = ops.images_to_cas_images(imgs, m).cpu().numpy()
+    assert rel_l2(ci2, oracle.imgs_to_cas_imgs(imgs.cpu().numpy(), m)) < 1e-5
+    bp = ops.back_project(_t(ci2, cuda), da, v, ker=dk)
+    bp_ref = oracle.back_project(ci2, axes, ker32, v)
+    assert_parity(bp.cpu().numpy(), bp_ref)
+    rec = ops.cas_volume_to_volume(bp, n, interp, pad)
+    assert rel_l2(rec.cpu().numpy(), oracle.cas_to_vol(bp_ref, n, interp, pad)) < 1e-5
