@@ -609,3 +609,35 @@ def test_other_interp_factors_and_paddings(cuda, oracle, ker32, n, interp, pad):
     assert_parity(bp.cpu().numpy(), bp_ref)
     rec = ops.cas_volume_to_volume(bp, n, interp, pad)
     assert rel_l2(rec.cpu().numpy(), oracle.cas_to_vol(bp_ref, n, interp, pad)) < 1e-5
+
+
+@pytest.mark.parametrize("r,r_int", [(0.0, 0.0), (0.7, 0.0), (1.0, 1.0), (-0.5, 0.0), (-1.0, -1.0)])
+def test_tiny_and_negative_mask_radii(cuda, oracle, ker32, r, r_int):
+    """Mask radii at the bottom of the range, through the int truncation of
+    RunKernel: 0 and 0.7 keep the centre pixel and voxel only, 1 their
+    neighbours too, -0.5 truncates to 0, and -1 masks everything out."""
+    import torch
+
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.synth import blob_volume, random_directions
+
+    n, p = 16, 6
+    m, v = 2 * n, 2 * n + 6
+    print(f"mask radius {r} (as {r_int})", flush=True)
+    cas = oracle.vol_to_cas(blob_volume(n, 5), 2, 3).astype(np.float32)
+    axes = random_directions(p, 14)
+    y = np.random.default_rng(14).standard_normal((p, m, m)).astype(np.float32)
+    dc, da, dy, dk = _t(cas, cuda), _t(axes, cuda), _t(y, cuda), _t(ker32, cuda)
+    fp = _np(ops.forward_project(dc, da, m, mask_radius=r, ker=dk))
+    fp_ref = oracle.forward_project(cas, axes, ker32, m, mask_radius=r_int)
+    assert np.count_nonzero(fp) == np.count_nonzero(fp_ref)
+    if fp_ref.any():
+        assert_parity(fp, fp_ref)
+    out = torch.zeros((v, v, v), device=cuda)
+    ops.back_project(dy, da, v, mask_radius=r, ker=dk, out=out)
+    bp_ref = oracle.back_project(y, axes, ker32, v, mask_radius=r_int)
+    assert np.count_nonzero(_np(out)) == np.count_nonzero(bp_ref)
+    if bp_ref.any():
+        assert_parity(_np(out), bp_ref)
+    if r_int < 0:
+        assert not fp.any() and not _np(out).any()
