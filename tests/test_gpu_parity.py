@@ -641,3 +641,34 @@ def test_tiny_and_negative_mask_radii(cuda, oracle, ker32, r, r_int):
         assert_parity(_np(out), bp_ref)
     if r_int < 0:
         assert not fp.any() and not _np(out).any()
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5])
+def test_tiny_volumes(cuda, oracle, ker32, n):
+    """CAS volumes far smaller than a pixel tile or a voxel brick (N = 1..5,
+    M = 2..10, V = 8..16): forward projection and back projection against the
+    oracle, and the image chain where M - N is even (the chains refuse an odd
+    difference)."""
+    from multigpugridder_amd import ops
+    from multigpugridder_amd.synth import random_directions
+
+    m, v = 2 * n, 2 * n + 6
+    print(f"tiny volume n={n} m={m} v={v}", flush=True)
+    # (a random CAS volume: the reference's precompensation, getPreComp.m, needs
+    # lines longer than the kernel's footprint)
+    cas = np.random.default_rng(n).standard_normal((v, v, v)).astype(np.float32)
+    axes = random_directions(5, 15)
+    dk, da = _t(ker32, cuda), _t(axes, cuda)
+    ci = ops.forward_project(_t(cas, cuda), da, m, ker=dk)
+    ci_ref = oracle.forward_project(cas, axes, ker32, m)
+    assert_parity(_np(ci), ci_ref)
+    if n % 2 == 0:
+        imgs = ops.cas_images_to_images(ci, n)
+        assert rel_l2(_np(imgs), oracle.cas_imgs_to_imgs(ci_ref, n)) < 1e-5
+    else:
+        from multigpugridder_amd import _lib
+        with pytest.raises(_lib.MggError):
+            ops.cas_images_to_images(ci, n)
+    y = np.random.default_rng(n + 1).standard_normal((5, m, m)).astype(np.float32)
+    bp = ops.back_project(_t(y, cuda), da, v, ker=dk)
+    assert_parity(_np(bp), oracle.back_project(y, axes, ker32, v))
